@@ -1,0 +1,193 @@
+/*
+ * synchrad_amd.h -- C ABI of the MI355X (gfx950) spectral radiation library.
+ *
+ * Every pointer named "device" is a HIP device pointer; every call is ordered
+ * on the `stream` argument (a hipStream_t, NULL = the default stream) and
+ * returns without synchronising.  The library never allocates: buffers are
+ * owned by the caller (torch tensors in the Python host) and borrowed for the
+ * duration of the call.  Functions return SR_OK (0) or a negative sr_status;
+ * sr_last_error() gives a message for the calling thread's last failure.
+ *
+ * Spectra are laid out (nSnaps, nPhi, nTheta|nRadius, nOmega) with omega
+ * fastest, as the reference's device buffers are (kernel_farfield.cl:102),
+ * and every entry point ADDS its contribution to them, as the reference's
+ * kernels do (`spectrum[...] +=`, kernel_farfield.cl:102).
+ */
+#ifndef SYNCHRAD_AMD_H
+#define SYNCHRAD_AMD_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define SR_ABI_VERSION 1
+
+typedef enum sr_status {
+  SR_OK = 0,
+  SR_ERR_INVALID = -1,     /* bad argument, shape or size */
+  SR_ERR_UNSUPPORTED = -2, /* valid request this build does not implement */
+  SR_ERR_HIP = -3,         /* a HIP runtime call failed */
+  SR_ERR_WORKSPACE = -4    /* workspace missing or too small */
+} sr_status;
+
+/* field mode, synchrad/calc.py:358-359 ('far' / 'near') */
+enum { SR_MODE_FAR = 0, SR_MODE_NEAR = 1 };
+
+/* vector components, synchrad/calc.py:458-465 */
+enum {
+  SR_COMP_TOTAL = 0,             /* 1 output:  total                         */
+  SR_COMP_CARTESIAN = 1,         /* 3 outputs: x, y, z                       */
+  SR_COMP_CARTESIAN_COMPLEX = 2, /* 6 outputs: xre, xim, yre, yim, zre, zim  */
+  SR_COMP_SPHERIC = 3,           /* 3 outputs: r, theta, phi   (far only)    */
+  SR_COMP_SPHERIC_COMPLEX = 4    /* 6 outputs: rre .. phiim    (far only)    */
+};
+
+/*
+ * Arithmetic.  SR_PREC_F64 is the reference's double kernels
+ * (calc_input['dtype'] = 'double').  SR_PREC_SPLIT keeps the retarded-time
+ * phase omega*(t - n.r) (far) or omega*(t + |P - r|) (near) in FP64, reduces
+ * it modulo 2*pi in FP64, and runs amplitudes, sin/cos and the compensated
+ * step sums in FP32 (calc_input['dtype'] = 'float').
+ */
+enum { SR_PREC_F64 = 0, SR_PREC_SPLIT = 1 };
+
+/*
+ * A batch of tracks packed CSR-style: particle p owns samples
+ * [offsets[p], offsets[p+1]) of the x..uz arrays.  Replaces the per-particle
+ * host loop of SynchRad.calculate_spectrum (synchrad/calc.py:257-267) and the
+ * per-track uploads of _track_to_device (synchrad/calc.py:579-603).
+ *
+ * Time alignment follows _process_track (synchrad/calc.py:296-307): for
+ * particle p the kernel runs iterations it < it_end[p]-1, the step is active
+ * while it < n_p - 1, the global iteration is it_start[p] + it, and snapshots
+ * are read from it_snaps + p*snaps_stride (snaps_stride = 0: one shared list,
+ * the global it_range mode; snaps_stride = n_snaps: one list per particle,
+ * the per-track mode where it_start = 0 and it_end = n_p).
+ */
+typedef struct sr_tracks {
+  int64_t n_particles;
+  int64_t n_points;            /* offsets[n_particles] */
+  const int64_t *offsets;      /* device [n_particles + 1] */
+  const int64_t *offsets_host; /* optional host copy, used to balance particle slices */
+  const double *x, *y, *z;     /* device [n_points], coordinates (length unit) */
+  const double *ux, *uy, *uz;  /* device [n_points], momenta (beta*gamma) */
+  const double *weight;        /* device [n_particles] */
+  const uint32_t *it_start;    /* device [n_particles] */
+  const uint32_t *it_end;      /* device [n_particles], >= 1 */
+  const uint32_t *it_snaps;    /* device, see above */
+  uint32_t n_snaps;            /* >= 1 */
+  uint32_t snaps_stride;       /* 0 or n_snaps */
+  double dt;                   /* timeStep = c*dt, synchrad/calc.py:125-127 */
+} sr_tracks;
+
+/*
+ * Spectral grid, as SynchRad._init_data uploads it (synchrad/calc.py:494-512).
+ * Far: omega (already multiplied by 2*pi), sin/cos of theta and phi.
+ * Near: omega, radius, sin/cos of phi and the screen distance.
+ */
+typedef struct sr_grid {
+  int32_t mode;             /* SR_MODE_FAR / SR_MODE_NEAR */
+  uint32_t n_omega;
+  uint32_t n_theta;         /* near: number of radii */
+  uint32_t n_phi;
+  const double *omega;      /* device [n_omega] */
+  const double *sin_theta;  /* device [n_theta]  (far) */
+  const double *cos_theta;  /* device [n_theta]  (far) */
+  const double *radius;     /* device [n_theta]  (near) */
+  const double *sin_phi;    /* device [n_phi] */
+  const double *cos_phi;    /* device [n_phi] */
+  double l_screen;          /* near: distance to the screen */
+} sr_grid;
+
+typedef struct sr_opts {
+  int32_t comp;             /* SR_COMP_* */
+  int32_t precision;        /* SR_PREC_* */
+  int32_t omega_per_lane;   /* 0 = tuned, else 1 or 2 (split) / 1 (f64) */
+  int32_t waves_per_block;  /* 0 = tuned, else 1, 2 or 4 */
+  int32_t slices;           /* 0 = tuned, else 1..16 particle slices */
+  int32_t persistent;       /* 0 = one block per tile; >0 = that many blocks loop over tiles */
+  const double *form_factor;/* device [n_omega]; far cartesian_complex only */
+} sr_opts;
+
+int sr_abi_version(void);
+const char *sr_last_error(void);
+
+/* Number of output arrays for (mode, comp): 1, 3 or 6; negative if invalid. */
+int sr_num_outputs(int32_t mode, int32_t comp);
+
+/*
+ * Workspace bytes sr_spectrum_batch needs for this request (0 when the
+ * chosen plan uses no particle slices).
+ */
+int sr_workspace_bytes(const sr_tracks *tracks, const sr_grid *grid, const sr_opts *opts,
+                       size_t *bytes);
+
+/*
+ * One spectrum of a whole CSR batch in one stream-ordered launch sequence
+ * (a tile kernel plus, with particle slices, a fixed-order slice reduction).
+ * Replaces SynchRad.calculate_spectrum's particle loop and its per-particle
+ * launches (synchrad/calc.py:257-267, 292-353).  `outs` holds
+ * sr_num_outputs() device arrays of nSnaps*n_phi*n_theta*n_omega doubles.
+ */
+int sr_spectrum_batch(const sr_tracks *tracks, const sr_grid *grid, const sr_opts *opts,
+                      double *const *outs, void *workspace, size_t workspace_bytes,
+                      void *stream);
+
+/*
+ * Per-particle entry points with the argument order of the reference's
+ * OpenCL kernels, plus a precision selector and a stream.  One call adds one
+ * particle's contribution, as one reference launch does.
+ */
+
+/* replaces far `total`, synchrad/kernel_farfield.cl:6-28 */
+int sr_far_total(double *spectrum,
+                 const double *x, const double *y, const double *z,
+                 const double *ux, const double *uy, const double *uz,
+                 double wp, uint32_t itStart, uint32_t itEnd, uint32_t nSteps,
+                 const double *omega, const double *sinTheta, const double *cosTheta,
+                 const double *sinPhi, const double *cosPhi,
+                 uint32_t nOmega, uint32_t nTheta, uint32_t nPhi,
+                 double dt, uint32_t nSnaps, const uint32_t *itSnaps,
+                 int32_t precision, void *stream);
+
+/* replaces far `cartesian_comps`, synchrad/kernel_farfield.cl:111-135 */
+int sr_far_cartesian(double *spectrum1, double *spectrum2, double *spectrum3,
+                     const double *x, const double *y, const double *z,
+                     const double *ux, const double *uy, const double *uz,
+                     double wp, uint32_t itStart, uint32_t itEnd, uint32_t nSteps,
+                     const double *omega, const double *sinTheta, const double *cosTheta,
+                     const double *sinPhi, const double *cosPhi,
+                     uint32_t nOmega, uint32_t nTheta, uint32_t nPhi,
+                     double dt, uint32_t nSnaps, const uint32_t *itSnaps,
+                     int32_t precision, void *stream);
+
+/* replaces near `total`, synchrad/kernel_nearfield.cl:5-27 */
+int sr_near_total(double *spectrum,
+                  const double *x, const double *y, const double *z,
+                  const double *ux, const double *uy, const double *uz,
+                  double wp, uint32_t itStart, uint32_t itEnd, uint32_t nSteps,
+                  const double *omega, const double *radius,
+                  const double *sinPhi, const double *cosPhi, double distanceToScreen,
+                  uint32_t nOmega, uint32_t nRadius, uint32_t nPhi,
+                  double dt, uint32_t nSnaps, const uint32_t *itSnaps,
+                  int32_t precision, void *stream);
+
+/* replaces near `cartesian_comps`, synchrad/kernel_nearfield.cl:106-130 */
+int sr_near_cartesian(double *spectrum1, double *spectrum2, double *spectrum3,
+                      const double *x, const double *y, const double *z,
+                      const double *ux, const double *uy, const double *uz,
+                      double wp, uint32_t itStart, uint32_t itEnd, uint32_t nSteps,
+                      const double *omega, const double *radius,
+                      const double *sinPhi, const double *cosPhi, double distanceToScreen,
+                      uint32_t nOmega, uint32_t nRadius, uint32_t nPhi,
+                      double dt, uint32_t nSnaps, const uint32_t *itSnaps,
+                      int32_t precision, void *stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* SYNCHRAD_AMD_H */

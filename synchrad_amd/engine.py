@@ -1,0 +1,320 @@
+"""Device-side engine: CSR track batches, spectral grids and the batched call.
+
+This is the layer ``SynchRad.calculate_spectrum`` sits on.  It replaces the
+reference's per-particle host loop (synchrad/calc.py:257-267), its per-track
+uploads (``_track_to_device``, synchrad/calc.py:579-603) and its per-particle
+launches (``_process_track``, synchrad/calc.py:292-353) with one upload of a
+CSR-packed batch and one stream-ordered library call per spectrum.
+
+Device buffers are torch tensors on the current HIP device; the library only
+borrows their pointers.
+"""
+from __future__ import annotations
+
+import ctypes
+from dataclasses import dataclass, field
+from typing import Dict, List, Optional, Sequence
+
+import numpy as np
+import torch
+
+from . import _abi
+
+PRECISIONS = {"double": _abi.SR_PREC_F64, "f64": _abi.SR_PREC_F64,
+              "float": _abi.SR_PREC_SPLIT, "split": _abi.SR_PREC_SPLIT}
+
+# synchrad/calc.py:458-465
+COMP_KEYS = {
+    "cartesian": ["x", "y", "z"],
+    "cartesian_complex": ["xre", "xim", "yre", "yim", "zre", "zim"],
+    "spheric": ["r", "theta", "phi"],
+    "spheric_complex": ["rre", "rim", "thetare", "thetaim", "phire", "phiim"],
+    "total": ["total"],
+}
+
+
+def _dev(device=None) -> torch.device:
+    if device is None:
+        if not torch.cuda.is_available():
+            raise _abi.SynchradError("no HIP device is visible; synchrad_amd runs on MI355X only")
+        return torch.device("cuda", torch.cuda.current_device())
+    return torch.device(device)
+
+
+def _ptr(t: Optional[torch.Tensor]):
+    return None if t is None else ctypes.c_void_p(t.data_ptr())
+
+
+def snap_iterations(it_range, nSnaps: int) -> np.ndarray:
+    """``np.linspace(it0, it1, nSnaps+1, dtype=uint32)[1:]``, synchrad/calc.py:626-630."""
+    return np.ascontiguousarray(
+        np.linspace(*(tuple(it_range) + (int(nSnaps) + 1,)), dtype=np.uint32)[1:])
+
+
+# ---------------------------------------------------------------------------
+# host packing
+# ---------------------------------------------------------------------------
+@dataclass
+class HostBatch:
+    """CSR-packed tracks on the host (numpy)."""
+    offsets: np.ndarray       # int64 [Np+1]
+    x: np.ndarray             # float64 [Npts] (also y, z, ux, uy, uz)
+    y: np.ndarray
+    z: np.ndarray
+    ux: np.ndarray
+    uy: np.ndarray
+    uz: np.ndarray
+    weight: np.ndarray        # float64 [Np]
+    it_start: np.ndarray      # uint32 [Np]
+    it_end: np.ndarray        # uint32 [Np]
+    it_snaps: np.ndarray      # uint32 [Np*nSnaps] or [nSnaps]
+    n_snaps: int
+    snaps_stride: int
+    dt: float
+
+    @property
+    def n_particles(self) -> int:
+        return int(self.offsets.size - 1)
+
+    @property
+    def n_points(self) -> int:
+        return int(self.offsets[-1])
+
+    def n_intervals(self) -> int:
+        """Sum over particles of the active step count min(n_p-1, it_end-1)."""
+        n = np.diff(self.offsets)
+        act = np.minimum(n - 1, self.it_end.astype(np.int64) - 1)
+        return int(np.clip(act, 0, None).sum())
+
+
+def pack_csr(offsets, x, y, z, ux, uy, uz, weight, it_start=None, *, dt: float,
+             it_range=None, nSnaps: int = 1) -> HostBatch:
+    """Pack already-concatenated tracks.  Time alignment as _process_track
+    (synchrad/calc.py:296-307): without ``it_range`` every particle runs its own
+    range (it_start = 0, it_end = n_p, snapshots over [0, n_p]); with it, all
+    particles share it_end = it_range[-1] and one snapshot list."""
+    offsets = np.ascontiguousarray(offsets, dtype=np.int64)
+    Np = offsets.size - 1
+    if Np < 0 or (Np > 0 and (offsets[0] != 0 or np.any(np.diff(offsets) < 0))):
+        raise ValueError("offsets must start at 0 and be non-decreasing")
+    arrs = [np.ascontiguousarray(a, dtype=np.float64) for a in (x, y, z, ux, uy, uz)]
+    for a in arrs:
+        if a.size != offsets[-1]:
+            raise ValueError("track arrays must have offsets[-1] samples")
+    weight = np.ascontiguousarray(np.broadcast_to(np.asarray(weight, dtype=np.float64), (Np,)))
+    nSnaps = int(nSnaps)
+    if nSnaps < 1:
+        raise ValueError("nSnaps must be >= 1")
+    lengths = np.diff(offsets)
+    if it_range is None:
+        its = np.zeros(Np, dtype=np.uint32)
+        ite = lengths.astype(np.uint32)
+        snaps = np.zeros((Np, nSnaps), dtype=np.uint32)
+        for n in np.unique(lengths):
+            snaps[lengths == n] = snap_iterations((0, int(n)), nSnaps)
+        snaps = snaps.reshape(-1)
+        stride = nSnaps
+    else:
+        it_range = tuple(int(v) for v in it_range)
+        if it_range[-1] <= 0:
+            raise ValueError("it_range end must be >= 1 (the reference loops 2^32 times on 0)")
+        its = (np.zeros(Np, dtype=np.uint32) if it_start is None
+               else np.ascontiguousarray(np.broadcast_to(np.asarray(it_start).astype(np.uint32), (Np,))))
+        ite = np.full(Np, it_range[-1], dtype=np.uint32)
+        snaps = snap_iterations(it_range, nSnaps)
+        stride = 0
+    if not np.isfinite(dt) or dt == 0:
+        raise ValueError("timeStep must be finite and non-zero")
+    return HostBatch(offsets, *arrs, weight, its, ite, np.ascontiguousarray(snaps, dtype=np.uint32),
+                     nSnaps, stride, float(dt))
+
+
+def pack_tracks(tracks: Sequence[Sequence], *, dt: float, it_range=None, nSnaps: int = 1) -> HostBatch:
+    """Pack reference-style track lists ``[x, y, z, ux, uy, uz, w, it_start]`` (or
+    the 7-item form, synchrad/calc.py:580-584)."""
+    Np = len(tracks)
+    lengths = np.array([np.asarray(t[0]).size for t in tracks], dtype=np.int64)
+    offsets = np.zeros(Np + 1, dtype=np.int64)
+    np.cumsum(lengths, out=offsets[1:])
+    cols = []
+    for c in range(6):
+        if Np:
+            cols.append(np.concatenate([np.asarray(t[c], dtype=np.float64).ravel() for t in tracks]))
+        else:
+            cols.append(np.zeros(0, dtype=np.float64))
+    w = np.array([float(t[6]) for t in tracks], dtype=np.float64)
+    its = np.array([int(np.uint32(t[7])) if len(t) == 8 else 0 for t in tracks], dtype=np.uint32)
+    return pack_csr(offsets, *cols, w, its, dt=dt, it_range=it_range, nSnaps=nSnaps)
+
+
+# ---------------------------------------------------------------------------
+# device objects
+# ---------------------------------------------------------------------------
+class DeviceBatch:
+    """A HostBatch uploaded to the HIP device (one copy per array)."""
+
+    def __init__(self, hb: HostBatch, device=None):
+        dev = _dev(device)
+        self.host = hb
+        self.device = dev
+
+        def f64(a):
+            return torch.from_numpy(np.ascontiguousarray(a, dtype=np.float64)).to(dev)
+
+        def u32(a):
+            return torch.from_numpy(np.ascontiguousarray(a, dtype=np.uint32).view(np.int32)).to(dev)
+
+        self.offsets = torch.from_numpy(hb.offsets).to(dev)
+        self.x, self.y, self.z = f64(hb.x), f64(hb.y), f64(hb.z)
+        self.ux, self.uy, self.uz = f64(hb.ux), f64(hb.uy), f64(hb.uz)
+        self.weight = f64(hb.weight)
+        self.it_start = u32(hb.it_start)
+        self.it_end = u32(hb.it_end)
+        self.it_snaps = u32(hb.it_snaps)
+        self._offsets_host = np.ascontiguousarray(hb.offsets, dtype=np.int64)
+
+    @property
+    def n_particles(self) -> int:
+        return self.host.n_particles
+
+    def abi(self) -> _abi.SrTracks:
+        hb = self.host
+        t = _abi.SrTracks()
+        t.n_particles = hb.n_particles
+        t.n_points = hb.n_points
+        t.offsets = self.offsets.data_ptr()
+        t.offsets_host = self._offsets_host.ctypes.data
+        for name in ("x", "y", "z", "ux", "uy", "uz"):
+            setattr(t, name, getattr(self, name).data_ptr())
+        t.weight = self.weight.data_ptr()
+        t.it_start = self.it_start.data_ptr()
+        t.it_end = self.it_end.data_ptr()
+        t.it_snaps = self.it_snaps.data_ptr()
+        t.n_snaps = hb.n_snaps
+        t.snaps_stride = hb.snaps_stride
+        t.dt = hb.dt
+        return t
+
+
+class DeviceGrid:
+    """Spectral axes on the device, formed as SynchRad._init_data forms them
+    (synchrad/calc.py:494-512): omega times 2*pi, and numpy sin/cos of the
+    double-precision angle axes."""
+
+    def __init__(self, mode: str, omega: np.ndarray, ax1: np.ndarray, phi: np.ndarray,
+                 L_screen: Optional[float] = None, device=None):
+        dev = _dev(device)
+        if mode not in _abi.MODES:
+            raise ValueError(f"mode must be 'far' or 'near', got {mode!r}")
+        self.mode = mode
+        self.device = dev
+        omega = np.asarray(omega, dtype=np.float64)
+        ax1 = np.asarray(ax1, dtype=np.float64)
+        phi = np.asarray(phi, dtype=np.float64)
+        self.n_omega, self.n1, self.n_phi = omega.size, ax1.size, phi.size
+
+        def f64(a):
+            return torch.from_numpy(np.ascontiguousarray(a, dtype=np.float64)).to(dev)
+
+        self.omega_host = omega
+        self.omega = f64(np.double(2 * np.pi) * omega)
+        self.sin_phi = f64(np.sin(phi))
+        self.cos_phi = f64(np.cos(phi))
+        self.sin_theta = self.cos_theta = self.radius = None
+        if mode == "far":
+            self.sin_theta = f64(np.sin(ax1))
+            self.cos_theta = f64(np.cos(ax1))
+            self.l_screen = 0.0
+        else:
+            if L_screen is None:
+                raise ValueError("near field needs L_screen")
+            self.radius = f64(ax1)
+            self.l_screen = float(L_screen)
+
+    @property
+    def n_grid(self) -> int:
+        return self.n_omega * self.n1 * self.n_phi
+
+    def abi(self) -> _abi.SrGrid:
+        g = _abi.SrGrid()
+        g.mode = _abi.MODES[self.mode]
+        g.n_omega, g.n_theta, g.n_phi = self.n_omega, self.n1, self.n_phi
+        g.omega = self.omega.data_ptr()
+        g.sin_phi = self.sin_phi.data_ptr()
+        g.cos_phi = self.cos_phi.data_ptr()
+        if self.mode == "far":
+            g.sin_theta = self.sin_theta.data_ptr()
+            g.cos_theta = self.cos_theta.data_ptr()
+        else:
+            g.radius = self.radius.data_ptr()
+        g.l_screen = self.l_screen
+        return g
+
+    def form_factor(self, sigma_particle: float) -> torch.Tensor:
+        """exp(-(2 pi omega sigma)^2 / 2), synchrad/calc.py:475-480."""
+        ff = np.exp(np.double(-0.5) * (np.double(2 * np.pi) * self.omega_host * np.double(sigma_particle)) ** 2)
+        return torch.from_numpy(np.ascontiguousarray(ff)).to(self.device)
+
+
+@dataclass
+class Tuning:
+    omega_per_lane: int = 0
+    waves_per_block: int = 0
+    slices: int = 0
+    persistent: int = 0
+
+
+def spectrum(batch: DeviceBatch, grid: DeviceGrid, comp: str = "total", precision="double",
+             outs: Optional[List[torch.Tensor]] = None, sigma_particle: float = 0.0,
+             tuning: Optional[Tuning] = None, stream=None) -> List[torch.Tensor]:
+    """Add one batch's spectrum to ``outs`` (allocated zero if None) and return
+    them.  Each output is float64 (nSnaps, nPhi, nTheta|nR, nOmega) on the
+    device -- the reference's device layout (kernel_farfield.cl:102).
+    Stream-ordered on the current (or given) torch stream; no synchronisation."""
+    L = _abi.load()
+    if comp not in _abi.COMPS:
+        raise ValueError(f"unknown comp {comp!r}")
+    prec = PRECISIONS[precision] if isinstance(precision, str) else int(precision)
+    n_out = L.sr_num_outputs(_abi.MODES[grid.mode], _abi.COMPS[comp])
+    if n_out < 0:
+        _abi.check(n_out)
+    shape = (batch.host.n_snaps, grid.n_phi, grid.n1, grid.n_omega)
+    if outs is None:
+        outs = [torch.zeros(shape, dtype=torch.float64, device=grid.device) for _ in range(n_out)]
+    if len(outs) != n_out:
+        raise ValueError(f"{comp} needs {n_out} output arrays")
+    for o in outs:
+        if tuple(o.shape) != shape or o.dtype != torch.float64 or not o.is_contiguous():
+            raise ValueError(f"outputs must be contiguous float64 of shape {shape}")
+    tun = tuning or Tuning()
+    opts = _abi.SrOpts()
+    opts.comp = _abi.COMPS[comp]
+    opts.precision = prec
+    opts.omega_per_lane = tun.omega_per_lane
+    opts.waves_per_block = tun.waves_per_block
+    opts.slices = tun.slices
+    opts.persistent = tun.persistent
+    ff = None
+    if comp == "cartesian_complex" and grid.mode == "far":
+        ff = grid.form_factor(sigma_particle)
+        opts.form_factor = ff.data_ptr()
+    tr = batch.abi()
+    gr = grid.abi()
+    nbytes = ctypes.c_size_t(0)
+    _abi.check(L.sr_workspace_bytes(ctypes.byref(tr), ctypes.byref(gr), ctypes.byref(opts),
+                                    ctypes.byref(nbytes)))
+    ws = None
+    if nbytes.value:
+        ws = torch.empty(nbytes.value, dtype=torch.uint8, device=grid.device)
+    out_ptrs = (ctypes.c_void_p * n_out)(*[o.data_ptr() for o in outs])
+    st = stream if stream is not None else torch.cuda.current_stream(grid.device)
+    _abi.check(L.sr_spectrum_batch(ctypes.byref(tr), ctypes.byref(gr), ctypes.byref(opts), out_ptrs,
+                                   None if ws is None else ctypes.c_void_p(ws.data_ptr()),
+                                   nbytes.value, ctypes.c_void_p(st.cuda_stream)))
+    # temporaries are freed to torch's stream-ordered caching allocator; tell
+    # it about the launch stream when that is not the one they were made on
+    if st != torch.cuda.current_stream(grid.device):
+        for t in (ff, ws):
+            if t is not None:
+                t.record_stream(st)
+    return outs

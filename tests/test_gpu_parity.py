@@ -1,0 +1,238 @@
+"""Parity of the HIP kernels (through the C ABI) with the CPU FP64 oracle.
+
+Bars: FP64 path <= 1e-10 relative L2 (and a max-element check), split path
+<= 1e-4 relative L2, both against the FP64 oracle on the same inputs.
+"""
+import ctypes
+
+import numpy as np
+import pytest
+import torch
+
+from oracle import oracle
+from synchrad_amd import _abi, engine, workloads
+
+pytestmark = pytest.mark.gpu
+
+F64_TOL = 1e-10
+SPLIT_TOL = 1e-4
+
+
+def rel_l2(got, ref):
+    d = np.linalg.norm((got - ref).ravel())
+    n = np.linalg.norm(ref.ravel())
+    return d / n if n > 0 else d
+
+
+def max_rel(got, ref, floor=1e-6):
+    m = np.abs(ref).max()
+    sel = np.abs(ref) > floor * m
+    return float(np.max(np.abs(got[sel] - ref[sel]) / np.abs(ref[sel]))) if sel.any() else 0.0
+
+
+def gpu_spectrum(tracks, axes, dt, mode="far", comp="total", precision="double", it_range=None,
+                 nSnaps=1, L_screen=None, sigma=0.0, tuning=None):
+    hb = engine.pack_tracks(tracks, dt=dt, it_range=it_range, nSnaps=nSnaps)
+    batch = engine.DeviceBatch(hb)
+    ax1 = axes["theta"] if mode == "far" else axes["radius"]
+    grid = engine.DeviceGrid(mode, axes["omega"], ax1, axes["phi"], L_screen=L_screen)
+    outs = engine.spectrum(batch, grid, comp=comp, precision=precision, sigma_particle=sigma,
+                           tuning=tuning)
+    torch.cuda.synchronize()
+    keys = engine.COMP_KEYS[comp]
+    return {k: np.ascontiguousarray(o.cpu().numpy().swapaxes(-1, -3)) for k, o in zip(keys, outs)}
+
+
+def check(got, ref, precision, label=""):
+    for k in ref:
+        r = rel_l2(got[k], ref[k])
+        print(f"{label} {k} {precision}: rel-L2 {r:.3e} max-rel {max_rel(got[k], ref[k]):.3e}", flush=True)
+        if precision == "double":
+            assert r <= F64_TOL, (k, r)
+        else:
+            assert r <= SPLIT_TOL, (k, r)
+
+
+@pytest.fixture(scope="module")
+def undulator():
+    wl = workloads.config1()
+    tr = wl.track_list()
+    ref = oracle.spectrum("far", "total", tr, wl.axes, wl.batch.dt)
+    return wl, tr, ref
+
+
+@pytest.mark.parametrize("precision", ["double", "float"])
+def test_far_total_undulator_full_config(undulator, precision):
+    """Single-electron undulator, 256 x 32 x 32 grid, 1600 steps."""
+    wl, tr, ref = undulator
+    got = gpu_spectrum(tr, wl.axes, wl.batch.dt, precision=precision)
+    check(got, ref, precision, "config1")
+    if precision == "double":
+        assert max_rel(got["total"], ref["total"]) < 1e-8
+
+
+@pytest.mark.parametrize("precision", ["double", "float"])
+def test_far_total_betatron_subset(precision):
+    wl = workloads.config2(n_particles=4, grid=(64, 16, 16))
+    tr = wl.track_list()
+    ref = oracle.spectrum("far", "total", tr, wl.axes, wl.batch.dt)
+    got = gpu_spectrum(tr, wl.axes, wl.batch.dt, precision=precision)
+    check(got, ref, precision, "betatron-4e")
+
+
+@pytest.mark.parametrize("comp", ["cartesian", "spheric", "cartesian_complex", "spheric_complex"])
+@pytest.mark.parametrize("precision", ["double", "float"])
+def test_far_components(comp, precision):
+    wl = workloads.config2(n_particles=3, n_steps=2000, grid=(48, 8, 8))
+    tr = wl.track_list()
+    sigma = 2e-11 if comp == "cartesian_complex" else 0.0
+    ref = oracle.spectrum("far", comp, tr, wl.axes, wl.batch.dt, sigma_particle=sigma)
+    got = gpu_spectrum(tr, wl.axes, wl.batch.dt, comp=comp, precision=precision, sigma=sigma)
+    if comp.endswith("complex") or comp == "spheric":
+        # components that are ~0 everywhere (the far-field radial part) are
+        # checked against the scale of the whole result instead of their own
+        scale = max(np.linalg.norm(ref[k]) for k in ref)
+        for k in ref:
+            r = np.linalg.norm(got[k] - ref[k]) / scale
+            print(f"{comp} {k} {precision}: err/scale {r:.3e}", flush=True)
+            assert r <= (F64_TOL if precision == "double" else SPLIT_TOL)
+    else:
+        check(got, ref, precision, comp)
+
+
+@pytest.mark.parametrize("comp", ["total", "cartesian", "cartesian_complex"])
+@pytest.mark.parametrize("precision", ["double", "float"])
+def test_near_field(comp, precision):
+    wl = workloads.config4(scale=0.25, n_particles=3)
+    tr = wl.track_list()
+    ref = oracle.spectrum("near", comp, tr, wl.axes, wl.batch.dt, L_screen=wl.L_screen)
+    got = gpu_spectrum(tr, wl.axes, wl.batch.dt, mode="near", comp=comp, precision=precision,
+                       L_screen=wl.L_screen)
+    if comp == "cartesian_complex":
+        scale = max(np.linalg.norm(ref[k]) for k in ref)
+        for k in ref:
+            r = np.linalg.norm(got[k] - ref[k]) / scale
+            print(f"near {comp} {k} {precision}: err/scale {r:.3e}", flush=True)
+            assert r <= (F64_TOL if precision == "double" else SPLIT_TOL)
+    else:
+        check(got, ref, precision, "near " + comp)
+
+
+def _ragged_tracks(n=7, seed=3):
+    """Short undulator tracks of different lengths, it_start offsets, one
+    single-sample track and one empty track."""
+    rng = np.random.default_rng(seed)
+    tr = []
+    lengths = [300, 1, 257, 0, 64, 513, 128][:n]
+    for i, L in enumerate(lengths):
+        g = rng.uniform(40, 80)
+        arrs = workloads.planar_undulator(g, rng.uniform(0.5, 1.5), 0.01, max(L, 0), 0.01 / 24,
+                                          psi=rng.uniform(0, 6.28))
+        tr.append(list(arrs) + [float(rng.uniform(0.5, 1.5)), int(rng.integers(0, 200))])
+    return tr
+
+
+@pytest.mark.parametrize("precision", ["double", "float"])
+@pytest.mark.parametrize("it_range,nSnaps", [(None, 1), (None, 3), ((0, 700), 4), ((100, 650), 3)])
+def test_ragged_batch_snapshots(precision, it_range, nSnaps):
+    tr = _ragged_tracks()
+    axes = oracle.make_axes([(100.0, 4e4), (0.0, 0.03), (0.0, 2 * np.pi), (40, 6, 5)], "far")
+    dt = 0.01 / 24
+    ref = oracle.spectrum("far", "total", tr, axes, dt, it_range=it_range, nSnaps=nSnaps)
+    got = gpu_spectrum(tr, axes, dt, precision=precision, it_range=it_range, nSnaps=nSnaps)
+    check(got, ref, precision, f"ragged it_range={it_range} nSnaps={nSnaps}")
+
+
+@pytest.mark.parametrize("precision", ["double", "float"])
+def test_uneven_csr_slices_agree(precision):
+    wl = workloads.config3(scale=0.25, n_particles=40)
+    tr = wl.track_list()
+    ref = oracle.spectrum("far", "cartesian", tr, wl.axes, wl.batch.dt)
+    one = gpu_spectrum(tr, wl.axes, wl.batch.dt, comp="cartesian", precision=precision,
+                       tuning=engine.Tuning(slices=1))
+    four = gpu_spectrum(tr, wl.axes, wl.batch.dt, comp="cartesian", precision=precision,
+                        tuning=engine.Tuning(slices=4))
+    check(one, ref, precision, "uneven slices=1")
+    for k in one:
+        assert rel_l2(four[k], one[k]) < 1e-13
+
+
+@pytest.mark.parametrize("tun", [engine.Tuning(waves_per_block=1), engine.Tuning(waves_per_block=2),
+                                 engine.Tuning(omega_per_lane=1, waves_per_block=4),
+                                 engine.Tuning(omega_per_lane=2, waves_per_block=2),
+                                 engine.Tuning(persistent=37)])
+def test_tile_shapes_agree(tun):
+    wl = workloads.config2(n_particles=2, n_steps=1500, grid=(200, 6, 4))
+    tr = wl.track_list()
+    ref = oracle.spectrum("far", "total", tr, wl.axes, wl.batch.dt)
+    for prec in ("double", "float"):
+        got = gpu_spectrum(tr, wl.axes, wl.batch.dt, precision=prec, tuning=tun)
+        check(got, ref, prec, f"tile {tun}")
+
+
+def test_per_particle_entry_points_match_batch():
+    """sr_far_total / sr_far_cartesian / sr_near_total (one call per particle,
+    the reference kernel ABI) add up to the batched result."""
+    L = _abi.load()
+    wl = workloads.config1(scale=0.125)
+    tr = _ragged_tracks(5, seed=9)
+    axes = oracle.make_axes([(100.0, 4e4), (0.0, 0.03), (0.0, 2 * np.pi), (40, 6, 5)], "far")
+    dt = 0.01 / 24
+    dev = torch.device("cuda")
+    f64 = lambda a: torch.as_tensor(np.ascontiguousarray(a, dtype=np.float64), device=dev)
+    om, st, ct = f64(2 * np.pi * axes["omega"]), f64(np.sin(axes["theta"])), f64(np.cos(axes["theta"]))
+    sp, cp = f64(np.sin(axes["phi"])), f64(np.cos(axes["phi"]))
+    nO, nT, nP = axes["omega"].size, axes["theta"].size, axes["phi"].size
+    for prec_name, prec in (("double", _abi.SR_PREC_F64), ("float", _abi.SR_PREC_SPLIT)):
+        tot = torch.zeros((1, nP, nT, nO), dtype=torch.float64, device=dev)
+        cart = [torch.zeros((1, nP, nT, nO), dtype=torch.float64, device=dev) for _ in range(3)]
+        for t in tr:
+            n = t[0].size
+            if n == 0:
+                continue
+            d = [f64(a) for a in t[:6]]
+            snaps = torch.as_tensor(oracle.snap_iterations((0, n), 1).view(np.int32), device=dev)
+            args = [c.data_ptr() for c in d] + [float(t[6]), 0, n, n]
+            ax = [om.data_ptr(), st.data_ptr(), ct.data_ptr(), sp.data_ptr(), cp.data_ptr(), nO, nT, nP,
+                  dt, 1, snaps.data_ptr(), prec, None]
+            _abi.check(L.sr_far_total(tot.data_ptr(), *args, *ax))
+            _abi.check(L.sr_far_cartesian(*[c.data_ptr() for c in cart], *args, *ax))
+        torch.cuda.synchronize()
+        ref_t = gpu_spectrum(tr, axes, dt, precision=prec_name)["total"]
+        ref_c = gpu_spectrum(tr, axes, dt, comp="cartesian", precision=prec_name)
+        assert rel_l2(tot.cpu().numpy().swapaxes(-1, -3), ref_t) < 1e-13
+        for k, c in zip("xyz", cart):
+            assert rel_l2(c.cpu().numpy().swapaxes(-1, -3), ref_c[k]) < 1e-13
+
+
+def test_near_per_particle_entry_point():
+    L = _abi.load()
+    wl = workloads.config4(scale=0.125, n_particles=2)
+    tr = wl.track_list()
+    axes = wl.axes
+    dev = torch.device("cuda")
+    f64 = lambda a: torch.as_tensor(np.ascontiguousarray(a, dtype=np.float64), device=dev)
+    nO, nR, nP = axes["omega"].size, axes["radius"].size, axes["phi"].size
+    ax = [f64(2 * np.pi * axes["omega"]), f64(axes["radius"]), f64(np.sin(axes["phi"])),
+          f64(np.cos(axes["phi"]))]
+    out = torch.zeros((1, nP, nR, nO), dtype=torch.float64, device=dev)
+    for t in tr:
+        n = t[0].size
+        d = [f64(a) for a in t[:6]]
+        snaps = torch.as_tensor(oracle.snap_iterations((0, n), 1).view(np.int32), device=dev)
+        _abi.check(L.sr_near_total(out.data_ptr(), *[c.data_ptr() for c in d], float(t[6]), 0, n, n,
+                                   *[a.data_ptr() for a in ax], wl.L_screen, nO, nR, nP, wl.batch.dt, 1,
+                                   snaps.data_ptr(), _abi.SR_PREC_F64, None))
+    torch.cuda.synchronize()
+    ref = oracle.spectrum("near", "total", tr, axes, wl.batch.dt, L_screen=wl.L_screen)["total"]
+    assert rel_l2(out.cpu().numpy().swapaxes(-1, -3), ref) <= F64_TOL
+
+
+def test_empty_and_degenerate_batches():
+    axes = oracle.make_axes([(100.0, 4e4), (0.0, 0.03), (0.0, 2 * np.pi), (16, 4, 4)], "far")
+    dt = 0.01 / 24
+    got = gpu_spectrum([], axes, dt)
+    assert np.all(got["total"] == 0)
+    z = np.zeros(1)
+    got = gpu_spectrum([[z, z, z, z, z, z + 10.0, 1.0, 0]], axes, dt)   # one sample: no interval
+    assert np.all(got["total"] == 0)

@@ -1,0 +1,415 @@
+"""Pins of the CPU oracle (oracle/oracle.c) against mathematics it must obey.
+
+The reference ships no golden vectors and its kernels cannot run here (no
+OpenCL device, no pyopencl), so each oracle function is pinned by results that
+do not come from retyping its formula:
+
+* Jackson's far-field integrand n x ((n - b) x a)/(1 - n.b)^2 (cross products)
+  for one-interval tracks, with the reference's |dphase| < pi filter;
+* zero radiation from uniform motion;
+* the undulator line: on-axis peak at w1 = 2 gamma^2 k_u/(1 + K^2/2) and the
+  peak height [w1 K N lambda_u (J0(xi) - J1(xi))/(2 gamma)]^2, xi = K^2/(4 + 2K^2);
+* the near field tends to the far field times 1/L^2 on axis as L grows;
+* the reference's own analytic photon-energy check
+  (tests/test_undulator_analytic.py:78-90 and its near twin);
+* invariants: linear in weight, additive over particles, |F|^2 preserved by the
+  spherical projection, zero radial component in the far field, coherent
+  outputs consistent with incoherent ones, snapshots = truncated tracks.
+
+test_planted_mistakes_are_caught builds deliberately broken copies of
+oracle.c and checks that every one of them fails at least one pin.
+"""
+import os
+import shutil
+import subprocess
+
+import numpy as np
+import pytest
+from scipy.special import jv
+
+from oracle import oracle
+from synchrad_amd import workloads
+from synchrad_amd.utils import J_in_um, Utilities
+
+RNG = np.random.default_rng(2024)
+
+
+def jackson_amp(n, b0, b1, dt):
+    """n x ((n - bbar) x a)/(1 - n.bbar)^2 with bbar, a from the step's end velocities."""
+    bbar = 0.5 * (b0 + b1)
+    a = (b1 - b0) / dt
+    return np.cross(n, np.cross(n - bbar, a)) / (1.0 - n @ bbar) ** 2
+
+
+def beta(u):
+    return u / np.sqrt(1.0 + u @ u)
+
+
+def dirs(theta, phi):
+    out = []
+    for p in phi:
+        for t in theta:
+            out.append(np.array([np.sin(t) * np.cos(p), np.sin(t) * np.sin(p), np.cos(t)]))
+    return out
+
+
+# --------------------------------------------------------------------------
+# pins, written so they can run against any build of oracle.c
+# --------------------------------------------------------------------------
+def pin_single_interval(library=None):
+    """One interval: S = w dt^2 |A_jackson|^2 when |phase_0| < pi, else 0."""
+    dt = 1e-3
+    u0 = np.array([0.7, -0.3, 40.0])
+    u1 = np.array([0.2, 0.4, 40.5])
+    x0 = np.array([2e-6, -1e-6, 3e-4])
+    tr = [[np.array([x0[0], x0[0] + 1e-6]), np.array([x0[1], x0[1]]), np.array([x0[2], x0[2] + dt]),
+           np.array([u0[0], u1[0]]), np.array([u0[1], u1[1]]), np.array([u0[2], u1[2]]), 1.7, 0]]
+    theta = np.array([0.0, 0.01, 0.03])
+    phi = np.array([0.0, 1.0, 2.5])
+    omega = np.array([10.0, 300.0, 2000.0, 9000.0])   # 1/lambda; phase = 2 pi omega tau
+    axes = {"omega": omega, "theta": theta, "phi": phi}
+    S = oracle.spectrum("far", "total", tr, axes, dt, library=library)["total"][0]
+    b0, b1 = beta(u0), beta(u1)
+    n_active = 0
+    for ip, p in enumerate(phi):
+        for it, t in enumerate(theta):
+            n = np.array([np.sin(t) * np.cos(p), np.sin(t) * np.sin(p), np.cos(t)])
+            A = jackson_amp(n, b0, b1, dt)
+            for io, om in enumerate(omega):
+                ph = 2 * np.pi * om * (0.0 - n @ x0)
+                want = 1.7 * dt * dt * (A @ A) if abs(ph) < np.pi else 0.0
+                n_active += abs(ph) < np.pi
+                got = S[io, it, ip]
+                if not np.isclose(got, want, rtol=1e-12, atol=0.0):
+                    return False
+    assert 0 < n_active < S.size  # both branches of the filter were exercised
+    return True
+
+
+def pin_filter_two_intervals(library=None):
+    """Two intervals: the second step is dropped when |phase_1 - phase_0| >= pi,
+    the first when |phase_0| >= pi (phase_{-1} = 0, kernel_farfield.cl:49,69-72)."""
+    dt = 0.01
+    u = np.array([[0.0, 0.0, 5.0], [0.5, 0.0, 5.0], [0.0, 0.3, 5.0]])
+    z = np.array([0.0, 0.002, 0.0])       # tau_s = s dt - z_s on axis
+    x = np.zeros(3)
+    tr = [[x, x.copy(), z, u[:, 0], u[:, 1], u[:, 2], 1.0, 0]]
+    omega = np.linspace(1.0, 90.0, 60)
+    axes = {"omega": omega, "theta": np.array([0.0]), "phi": np.array([0.0])}
+    S = oracle.spectrum("far", "total", tr, axes, dt, library=library)["total"][0, :, 0, 0]
+    n = np.array([0.0, 0.0, 1.0])
+    bs = [beta(v) for v in u]
+    A = [jackson_amp(n, bs[s], bs[s + 1], dt) for s in range(2)]
+    tau = np.arange(3) * dt - z
+    seen = set()
+    for io, om in enumerate(omega):
+        w = 2 * np.pi * om
+        ph = w * tau
+        act0 = abs(ph[0]) < np.pi
+        act1 = abs(ph[1] - ph[0]) < np.pi
+        seen.add((act0, act1))
+        F = np.zeros(3, dtype=complex)
+        if act0:
+            F += A[0] * np.exp(1j * ph[0])
+        if act1:
+            F += A[1] * np.exp(1j * ph[1])
+        want = dt * dt * np.sum(np.abs(F) ** 2)
+        if not np.isclose(S[io], want, rtol=1e-11, atol=1e-300):
+            return False
+    assert len(seen) >= 2
+    return True
+
+
+def pin_uniform_motion(library=None):
+    n = 50
+    dt = 1e-3
+    u = np.array([0.3, -0.1, 20.0])
+    b = beta(u)
+    s = np.arange(n) * dt
+    tr = [[b[0] * s, b[1] * s, b[2] * s, np.full(n, u[0]), np.full(n, u[1]), np.full(n, u[2]), 1.0, 0]]
+    axes = {"omega": np.array([5.0, 50.0]), "theta": np.array([0.0, 0.05]), "phi": np.array([0.0, 1.0])}
+    S = oracle.spectrum("far", "total", tr, axes, dt, library=library)["total"]
+    return bool(np.all(S == 0.0))
+
+
+def pin_undulator_line(library=None):
+    """Peak position and Bessel-function peak height of the config-1 undulator."""
+    wl = workloads.config1()
+    g, K, lu, N = 100.0, 1.0, 0.01, 50
+    w1 = wl.extra["omega1"]
+    omega = np.linspace(0.97, 1.03, 31) * w1
+    axes = {"omega": omega, "theta": np.array([0.0]), "phi": np.array([0.0])}
+    S = oracle.spectrum("far", "total", wl.track_list(), axes, wl.batch.dt,
+                        library=library)["total"][0, :, 0, 0]
+    if abs(omega[np.argmax(S)] / w1 - 1.0) > 1.0 / (4 * N):
+        return False
+    xi = K ** 2 / (4 + 2 * K ** 2)
+    peak = (2 * np.pi * w1 * K * N * lu * (jv(0, xi) - jv(1, xi)) / (2 * g)) ** 2
+    return abs(S.max() / peak - 1.0) < 0.02
+
+
+def consistent_undulator(K, g, lu, periods, spp, ramp=2.0):
+    """Ramped planar undulator whose positions integrate its velocities
+    (positions staggered by dt/2 as in the reference's examples)."""
+    ku = 2 * np.pi / lu
+    dt = lu / spp
+    n = int(periods * spp) + 1
+    s = np.arange(n) * dt
+    L = periods * lu
+    env = np.clip(np.minimum(s, L - s) / (ramp * lu), 0, 1)
+    env = env * env * (3 - 2 * env)
+    ux = K * env * np.cos(ku * s)
+    uy = np.zeros_like(s)
+    uz = np.sqrt(g * g - 1 - ux ** 2)
+    gam = np.sqrt(1 + ux ** 2 + uy ** 2 + uz ** 2)
+    b = np.stack([ux, uy, uz]) / gam
+    pos = np.zeros((3, n))
+    pos[:, 1:] = np.cumsum(0.5 * (b[:, 1:] + b[:, :-1]) * dt, axis=1)
+    pos = pos + 0.5 * dt * b
+    return [pos[0], pos[1], pos[2], ux, uy, uz, 1.0, 0], dt
+
+
+def pin_near_to_far(library=None):
+    """|F_near|^2 L^2 -> |F_far|^2 on axis for L >> undulator length."""
+    tr, dt = consistent_undulator(1.0, 100.0, 0.01, 30, 32)
+    w1 = 2 * 100.0 ** 2 / (0.01 * 1.5)
+    omega = np.linspace(0.9, 1.1, 11) * w1
+    far = oracle.spectrum("far", "total", [tr], {"omega": omega, "theta": np.array([0.0]),
+                                                 "phi": np.array([0.0])}, dt,
+                          library=library)["total"][0, :, 0, 0]
+    L = 1e3
+    near = oracle.spectrum("near", "total", [tr], {"omega": omega, "radius": np.array([0.0]),
+                                                   "phi": np.array([0.0])}, dt, L_screen=L,
+                           library=library)["total"][0, :, 0, 0]
+    sel = far > 0.1 * far.max()
+    return bool(np.all(np.abs(near[sel] * L * L / far[sel] - 1.0) < 0.03))
+
+
+def pin_near_potentials(library=None):
+    """Near field from the potentials: F = -grad_P Phi + i w A with
+    Phi = sum_s e^{i w (t_s + R_s)}/R_s and A = sum_s b_s e^{i w (t_s + R_s)}/R_s,
+    R_s = |P - x_s|; the gradient is taken by central differences in P.  This
+    weighs the Coulomb term n/R^2 against the radiation term i w (b - n)/R."""
+    dt = 0.01
+    n = 40
+    s = np.arange(n) * dt
+    x = 0.05 * np.sin(7.0 * s)
+    y = 0.03 * np.cos(5.0 * s) - 0.03
+    z = 0.6 * s
+    vel = np.stack([0.05 * 7.0 * np.cos(7.0 * s), -0.03 * 5.0 * np.sin(5.0 * s), np.full(n, 0.6)])
+    b = vel
+    u = b / np.sqrt(1.0 - np.sum(b * b, axis=0))
+    tr = [[x, y, z, u[0], u[1], u[2], 1.3, 0]]
+    omega = np.array([0.05, 0.15, 0.3])          # 1/lambda; w R < pi keeps every step active
+    radius = np.array([0.1, 0.5])
+    phi = np.array([0.3, 2.0])
+    L = 1.2
+    axes = {"omega": omega, "radius": radius, "phi": phi}
+    cart = oracle.spectrum("near", "cartesian", tr, axes, dt, L_screen=L, library=library)
+    tot = oracle.spectrum("near", "total", tr, axes, dt, L_screen=L, library=library)["total"]
+    # steps 0 .. n-2 contribute (the last sample only closes the last interval)
+    pos = np.stack([x, y, z])[:, :-1]
+    b = b[:, :-1]
+    s = s[:-1]
+    h = 1e-5
+    for io, om in enumerate(omega):
+        w = 2 * np.pi * om
+        for ir, r in enumerate(radius):
+            for ip, ph in enumerate(phi):
+                P = np.array([r * np.cos(ph), r * np.sin(ph), L])
+
+                def Phi(Q):
+                    R = np.linalg.norm(Q[:, None] - pos, axis=0)
+                    return np.sum(np.exp(1j * w * (s + R)) / R)
+
+                R = np.linalg.norm(P[:, None] - pos, axis=0)
+                A = np.sum(b * np.exp(1j * w * (s + R)) / R, axis=1)
+                grad = np.array([(Phi(P + h * e) - Phi(P - h * e)) / (2 * h) for e in np.eye(3)])
+                F = -grad + 1j * w * A
+                want = 1.3 * dt * dt * np.abs(F) ** 2
+                got = np.array([cart[c][0, io, ir, ip] for c in "xyz"])
+                if not np.allclose(got, want, rtol=1e-6, atol=1e-9 * want.max()):
+                    return False
+                if not np.isclose(tot[0, io, ir, ip], want.sum(), rtol=1e-6):
+                    return False
+    return True
+
+
+def pin_spheric_and_complex(library=None):
+    """total = sum of spheric components; radial far-field component = 0;
+    coherent components of one w=1 particle give |F_c|^2 = cartesian."""
+    wl = workloads.config1(scale=0.0625)
+    tr = wl.track_list()
+    ax = wl.axes
+    dt = wl.batch.dt
+    tot = oracle.spectrum("far", "total", tr, ax, dt, library=library)["total"]
+    sph = oracle.spectrum("far", "spheric", tr, ax, dt, library=library)
+    car = oracle.spectrum("far", "cartesian", tr, ax, dt, library=library)
+    cc = oracle.spectrum("far", "cartesian_complex", tr, ax, dt, library=library)
+    scale = np.abs(tot).max()
+    ok = np.allclose(sph["r"] + sph["theta"] + sph["phi"], tot, rtol=1e-9, atol=1e-12 * scale)
+    ok &= np.abs(sph["r"]).max() < 1e-12 * scale
+    ok &= np.allclose(car["x"] + car["y"] + car["z"], tot, rtol=1e-12, atol=1e-14 * scale)
+    for c in "xyz":
+        ok &= np.allclose(cc[c + "re"] ** 2 + cc[c + "im"] ** 2, car[c], rtol=1e-9,
+                          atol=1e-12 * scale)
+    return bool(ok)
+
+
+def pin_snapshots_are_truncations(library=None):
+    """With it_range=(0, n) and 2 snapshots, snapshot 0 equals the spectrum of
+    the track cut after the record iteration, snapshot 1 the full track."""
+    wl = workloads.config1(scale=0.0625)
+    (x, y, z, ux, uy, uz, w, its), = wl.track_list()
+    n = x.size
+    ax = wl.axes
+    dt = wl.batch.dt
+    snaps = oracle.snap_iterations((0, n), 2)
+    two = oracle.spectrum("far", "total", [[x, y, z, ux, uy, uz, w, 0]], ax, dt, it_range=(0, n),
+                          nSnaps=2, library=library)["total"]
+    r0 = int(snaps[0]) - 2          # record after iteration r0: steps 0..r0 -> r0 + 2 points
+    cut = oracle.spectrum("far", "total", [[a[:r0 + 2] for a in (x, y, z, ux, uy, uz)] + [w, 0]],
+                          ax, dt, library=library)["total"]
+    full = oracle.spectrum("far", "total", [[x, y, z, ux, uy, uz, w, 0]], ax, dt,
+                           library=library)["total"]
+    return bool(np.array_equal(two[0], cut[0]) and np.array_equal(two[1], full[0]))
+
+
+ALL_PINS = [pin_single_interval, pin_filter_two_intervals, pin_uniform_motion, pin_undulator_line,
+            pin_near_to_far, pin_near_potentials, pin_spheric_and_complex,
+            pin_snapshots_are_truncations]
+
+
+@pytest.mark.parametrize("pin", ALL_PINS, ids=lambda f: f.__name__)
+def test_pin(pin):
+    assert pin()
+
+
+# --------------------------------------------------------------------------
+# invariants and the reference's own analytic checks
+# --------------------------------------------------------------------------
+def test_linear_in_weight_and_additive_over_particles():
+    wl = workloads.config2(n_particles=3, n_steps=300, grid=(16, 6, 4))
+    tr = wl.track_list()
+    ax = wl.axes
+    dt = wl.batch.dt
+    S_all = oracle.spectrum("far", "total", tr, ax, dt)["total"]
+    parts = [oracle.spectrum("far", "total", [t], ax, dt)["total"] for t in tr]
+    assert np.allclose(S_all, sum(parts), rtol=1e-13, atol=0)
+    t2 = [list(t) for t in tr]
+    for t in t2:
+        t[6] = 2.0 * t[6]
+    S2 = oracle.spectrum("far", "total", t2, ax, dt)["total"]
+    assert np.array_equal(S2, 2.0 * S_all)
+    S_rev = oracle.spectrum("far", "total", tr[::-1], ax, dt)["total"]
+    assert np.allclose(S_rev, S_all, rtol=1e-13, atol=0)
+
+
+def _reference_script_tracks(SPP, Np=2, seed=11):
+    """The orbit of tests/test_undulator_analytic.py:7-50 (seeded)."""
+    K0, Periods, g0 = 0.1, 50, 100.0
+    dg = 1e-4 * g0
+    gg = g0 / (1.0 + K0 ** 2 / 2) ** 0.5
+    k_res = 2 * gg ** 2
+    dt = 1.0 / SPP
+    steps = int((Periods + 2) / dt) + 1
+
+    def uxf(z):
+        val = K0 * np.sin(2 * np.pi * z)
+        val *= (z > 0) * (z < 1.5) * z / 1.5 + (z > 1.5)
+        val *= (z > Periods - 1.5) * (z < Periods) * (Periods - z) / 1.5 + (z < Periods - 1.5)
+        return val
+
+    t = np.r_[-1:Periods + 1:steps * 1j]
+    rng = np.random.default_rng(seed)
+    tracks = []
+    for g0p in g0 + dg * rng.standard_normal(Np):
+        ggp = g0p / (1.0 + K0 ** 2 / 2) ** 0.5
+        vb = (1.0 - ggp ** -2) ** 0.5
+        z = vb * t
+        ux = uxf(z - 0.5 * dt)
+        uz = (g0 ** 2 - 1 - ux ** 2) ** 0.5
+        x = ux[0] / g0p * dt / 2 + np.cumsum(ux / g0p) * dt
+        tracks.append([x, np.zeros_like(t), z, ux, np.zeros_like(t), uz, 1.0, 0])
+    e_theory = Np * k_res * J_in_um * (7 * np.pi / 24) / 137.0 * K0 ** 2 * (1 + K0 ** 2 / 2) * Periods
+    return tracks, dt, k_res, e_theory
+
+
+class _Spec(Utilities):
+    def __init__(self, rad, mode, axes):
+        self.Data = {"radiation": rad}
+        self.Args = {"mode": mode, "comp": "total", "omega": axes["omega"], "phi": axes["phi"],
+                     "dph": axes["phi"][1] - axes["phi"][0]}
+        self.Args.update({k: v for k, v in axes.items() if k in ("theta", "radius")})
+
+
+def test_reference_analytic_energy_far():
+    """tests/test_undulator_analytic.py:66-90: the far-field photon energy of a
+    K=0.1 undulator against (7 pi/24) alpha K^2 (1 + K^2/2) N k_res J.  The
+    oracle lands at ~1% (24 particles: 1.06%); asserted below 3%."""
+    tracks, dt, k_res, e_th = _reference_script_tracks(32)
+    grid = [(0.02 * k_res, 1.1 * k_res), (0, 2.0 / 100.0), (0.0, 2 * np.pi), (128, 32, 32)]
+    axes = oracle.make_axes(grid, "far")
+    rad = oracle.spectrum("far", "total", tracks, axes, dt)
+    e = _Spec(rad, "far", axes).get_energy(lambda0_um=1)
+    assert abs(e - e_th) / e_th < 0.03
+
+
+def test_reference_analytic_energy_near_reduced():
+    """tests/test_undulator_analytic_near.py on a coarser screen (96 radii x 8
+    phi instead of 256 x 32, one particle): the near-field energy at L = 1e5
+    lambda_u within 15% of the same estimate (the full grid gives 10.9%)."""
+    tracks, dt, k_res, e_th = _reference_script_tracks(64, Np=1)
+    L_scr = 1e5
+    grid = [(0.02 * k_res, 1.1 * k_res), (0, L_scr / 100.0), (0.0, 2 * np.pi), (64, 96, 8)]
+    axes = oracle.make_axes(grid, "near")
+    rad = oracle.spectrum("near", "total", tracks, axes, dt, L_screen=L_scr)
+    e = _Spec(rad, "near", axes).get_energy(lambda0_um=1)
+    assert abs(e - e_th) / e_th < 0.15
+
+
+# --------------------------------------------------------------------------
+# planted mistakes: each broken copy of oracle.c must fail some pin
+# --------------------------------------------------------------------------
+MUTANTS = {
+    "drop_c1_term": ("v3 amp = sub3(scl3(c1, sub3(nVec, ub)), scl3(c2, a));",
+                     "v3 amp = scl3(-c2, a);"),
+    "phase_sign": ("const double phase = omegaLocal * (time - dot3(xl, nVec));",
+                   "const double phase = omegaLocal * (time + dot3(xl, nVec));"),
+    "c2_not_squared": ("c1 = c1 * c2 * c2;", "c1 = c1 * c2;"),
+    "no_velocity_average": ("const v3 ub = scl3(0.5, add3(un, u));", "const v3 ub = u;"),
+    "filter_inverted": ("if (dPhase < M_PI) {\n          /* kernel_farfield.cl:74-96 */",
+                        "if (dPhase >= M_PI) {\n          /* kernel_farfield.cl:74-96 */"),
+    "near_omega_dropped": ("const v3 c1 = scl3(omegaLocal * rInv, sub3(u, nVec));",
+                           "const v3 c1 = scl3(rInv, sub3(u, nVec));"),
+    "near_r_power": ("const v3 c2 = scl3(rInv * rInv, nVec);", "const v3 c2 = scl3(rInv, nVec);"),
+    "near_c1_sign": ("re = add3(re, add3(scl3(-sinPhase, c1), scl3(cosPhase, c2)));",
+                     "re = add3(re, add3(scl3(sinPhase, c1), scl3(cosPhase, c2)));"),
+    "spheric_theta_sign": ("const v3 thVec = mk(cosTheta[iTheta] * cosPhi[iPhi], cosTheta[iTheta] * sinPhi[iPhi],\n                        -sinTheta[iTheta]);",
+                           "const v3 thVec = mk(cosTheta[iTheta] * cosPhi[iPhi], cosTheta[iTheta] * sinPhi[iPhi],\n                        sinTheta[iTheta]);"),
+    "snapshot_off_by_one": ("it_glob + 2 == itSnaps[iSnap]) {\n        record(comp, spectra, (size_t)(gti + nTotal * iSnap), wpdt2, wpdt, re, im);\n        iSnap += 1;\n      }\n    }\n  }\n  return 0;\n}\n\n/*",
+                            "it_glob + 1 == itSnaps[iSnap]) {\n        record(comp, spectra, (size_t)(gti + nTotal * iSnap), wpdt2, wpdt, re, im);\n        iSnap += 1;\n      }\n    }\n  }\n  return 0;\n}\n\n/*"),
+}
+
+
+@pytest.mark.skipif(shutil.which("gcc") is None, reason="needs gcc")
+@pytest.mark.parametrize("name", sorted(MUTANTS))
+def test_planted_mistakes_are_caught(name, tmp_path):
+    src = open(os.path.join(os.path.dirname(oracle.__file__), "oracle.c")).read()
+    old, new = MUTANTS[name]
+    assert src.count(old) == 1, f"mutation site for {name} not found exactly once"
+    mut = tmp_path / "oracle_mut.c"
+    mut.write_text(src.replace(old, new))
+    so = tmp_path / "liboracle_mut.so"
+    subprocess.run(["gcc", "-O2", "-fPIC", "-fopenmp", "-ffp-contract=off", "-shared", "-o", str(so),
+                    str(mut), "-lm"], check=True)
+    L = oracle.lib(str(so))
+    failed = [p.__name__ for p in ALL_PINS if not _safe(p, L)]
+    assert failed, f"mutant {name} passed every pin"
+
+
+def _safe(pin, L):
+    try:
+        return pin(L)
+    except (AssertionError, FloatingPointError, ValueError):
+        return False
