@@ -385,10 +385,13 @@ __global__ void __launch_bounds__(kMaxT) k_spectrum(const KParams p) {
                   const float f = (float)(rv - rint(rv));
                   const float sn = __builtin_amdgcn_sinf(f);
                   const float cn = __builtin_amdgcn_cosf(f);
+                  // word layout shared with the FP64 path: per amplitude vector
+                  // v (far: A; near: v1 then v2) [v cos | v sin], 3 words each
 #pragma unroll
                   for (int c = 0; c < NA; c++) {
-                    ps[k][c] = __builtin_fmaf((float)a[c], cn, ps[k][c]);
-                    ps[k][NA + c] = __builtin_fmaf((float)a[c], sn, ps[k][NA + c]);
+                    const int w = 6 * (c / 3) + (c % 3);
+                    ps[k][w] = __builtin_fmaf((float)a[c], cn, ps[k][w]);
+                    ps[k][w + 3] = __builtin_fmaf((float)a[c], sn, ps[k][w + 3]);
                   }
                 } else {
                   double sn, cn;

@@ -101,7 +101,7 @@ def pack_csr(offsets, x, y, z, ux, uy, uz, weight, it_start=None, *, dt: float,
     for a in arrs:
         if a.size != offsets[-1]:
             raise ValueError("track arrays must have offsets[-1] samples")
-    weight = np.ascontiguousarray(np.broadcast_to(np.asarray(weight, dtype=np.float64), (Np,)))
+    weight = np.array(np.broadcast_to(np.asarray(weight, dtype=np.float64), (Np,)))
     nSnaps = int(nSnaps)
     if nSnaps < 1:
         raise ValueError("nSnaps must be >= 1")
@@ -119,7 +119,7 @@ def pack_csr(offsets, x, y, z, ux, uy, uz, weight, it_start=None, *, dt: float,
         if it_range[-1] <= 0:
             raise ValueError("it_range end must be >= 1 (the reference loops 2^32 times on 0)")
         its = (np.zeros(Np, dtype=np.uint32) if it_start is None
-               else np.ascontiguousarray(np.broadcast_to(np.asarray(it_start).astype(np.uint32), (Np,))))
+               else np.array(np.broadcast_to(np.asarray(it_start).astype(np.uint32), (Np,))))
         ite = np.full(Np, it_range[-1], dtype=np.uint32)
         snaps = snap_iterations(it_range, nSnaps)
         stride = 0

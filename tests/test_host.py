@@ -1,0 +1,181 @@
+"""Host-side logic that needs no GPU: axes, CSR packing, snapshot schedules,
+weight normalisation, analysis helpers and the benchmark workloads."""
+import numpy as np
+import pytest
+
+from oracle import oracle
+from synchrad.calc import SynchRad
+from synchrad_amd import engine, workloads
+from synchrad_amd.utils import J_in_um, Utilities, alpha_fs
+
+
+@pytest.mark.parametrize("mode", ["far", "near"])
+@pytest.mark.parametrize("features", [[], ["logGrid"], ["wavelengthGrid"]])
+def test_axes_match_reference_construction(mode, features):
+    grid = [(1e3, 7e5), (0.0, 0.05), (0.3, 2 * np.pi), (37, 11, 9)]
+    calc = SynchRad({"grid": grid, "mode": mode, "ctx": False, "Features": list(features)})
+    ref = oracle.make_axes(grid, mode, features)
+    assert np.array_equal(calc.Args["omega"], ref["omega"])
+    assert np.array_equal(calc.Args["phi"], ref["phi"])
+    key = "theta" if mode == "far" else "radius"
+    assert np.array_equal(calc.Args[key], ref[key])
+    # phi is endpoint-exclusive, theta/radius inclusive (synchrad/calc.py:413-414, 436-437)
+    assert calc.Args["phi"][-1] < 2 * np.pi and calc.Args[key][-1] == 0.05
+    assert calc.Args["dph"] == pytest.approx((2 * np.pi - 0.3) / 9)
+    assert calc.Args["dV"].shape == (36,)
+
+
+def test_dtype_selects_precision():
+    grid = [(1.0, 2.0), (0.0, 0.1), (0.0, 6.28), (4, 4, 4)]
+    assert SynchRad({"grid": grid, "ctx": False}).precision == "double"
+    c = SynchRad({"grid": grid, "ctx": False, "dtype": "float"})
+    assert c.precision == "float" and c.Args["omega"].dtype == np.float64
+    with pytest.raises(ValueError):
+        SynchRad({"grid": grid, "ctx": False, "dtype": "half"})
+
+
+def test_per_track_snapshots_equal_per_particle_linspace():
+    rng = np.random.default_rng(0)
+    lengths = rng.integers(1, 60, 25)
+    tracks = [[np.zeros(n)] * 6 + [1.0, 0] for n in lengths]
+    for nSnaps in (1, 2, 5, 7):
+        hb = engine.pack_tracks(tracks, dt=0.1, nSnaps=nSnaps)
+        assert hb.snaps_stride == nSnaps
+        got = hb.it_snaps.reshape(len(tracks), nSnaps)
+        for p, n in enumerate(lengths):
+            want = np.linspace(0, int(n), nSnaps + 1, dtype=np.uint32)[1:]
+            assert np.array_equal(got[p], want)
+        assert np.array_equal(hb.it_end, lengths.astype(np.uint32))
+        assert np.all(hb.it_start == 0)
+
+
+def test_global_it_range_packing():
+    tracks = [[np.zeros(10)] * 6 + [2.0, 7], [np.zeros(4)] * 6 + [0.5, 3], [np.zeros(3)] * 6 + [1.0]]
+    hb = engine.pack_tracks(tracks, dt=0.1, it_range=(2, 20), nSnaps=3)
+    assert hb.snaps_stride == 0
+    assert np.array_equal(hb.it_snaps, np.linspace(2, 20, 4, dtype=np.uint32)[1:])
+    assert list(hb.it_start) == [7, 3, 0]
+    assert list(hb.it_end) == [20, 20, 20]
+    assert list(hb.offsets) == [0, 10, 14, 17]
+    assert list(hb.weight) == [2.0, 0.5, 1.0]
+    with pytest.raises(ValueError):
+        engine.pack_tracks(tracks, dt=0.1, it_range=(0, 0))
+    with pytest.raises(ValueError):
+        engine.pack_tracks(tracks, dt=0.0)
+
+
+def test_n_intervals_counts_active_steps():
+    tracks = [[np.zeros(n)] * 6 + [1.0, 0] for n in (0, 1, 2, 50)]
+    assert engine.pack_tracks(tracks, dt=1.0).n_intervals() == 0 + 0 + 1 + 49
+    # a global window shorter than a track stops its loop (kernel_farfield.cl:59)
+    assert engine.pack_tracks(tracks, dt=1.0, it_range=(0, 10)).n_intervals() == 0 + 0 + 1 + 9
+
+
+def test_weight_normalisation_mutates_caller_lists(monkeypatch):
+    """synchrad/calc.py:241-265: 'mean'/'max' divide and 'ones' overwrite the
+    weights in the caller's own track lists; total_weight sums them."""
+    grid = [(1.0, 2.0), (0.0, 0.1), (0.0, 6.28), (4, 4, 4)]
+    calc = SynchRad({"grid": grid, "ctx": False})
+    calc.device = "cpu-test"
+    seen = {}
+
+    def fake_spectrum(batch, grid, **kw):
+        seen["w"] = batch.host.weight.copy()
+        return kw["outs"]
+
+    class FakeBatch:
+        def __init__(self, hb, device):
+            self.host = hb
+
+    monkeypatch.setattr(engine, "DeviceBatch", FakeBatch)
+    monkeypatch.setattr(engine, "spectrum", fake_spectrum)
+    monkeypatch.setattr(SynchRad, "_init_raditaion", lambda self, comp, n: setattr(self, "_rad_keys", []))
+    monkeypatch.setattr(SynchRad, "_spectr_from_device", lambda self, n: None)
+    calc._rad_device = {}
+    calc._grid = None
+    tracks = [[np.zeros(3)] * 6 + [w, 0] for w in (1.0, 2.0, 6.0)]
+    calc.calculate_spectrum(tracks, timeStep=0.1, weights_normalize="mean", verbose=False)
+    assert [t[6] for t in tracks] == [1.0 / 3, 2.0 / 3, 2.0]
+    assert calc.total_weight == pytest.approx(3.0)
+    assert np.allclose(seen["w"], [1.0 / 3, 2.0 / 3, 2.0])
+    calc.calculate_spectrum(tracks, timeStep=0.1, weights_normalize="ones", verbose=False)
+    assert [t[6] for t in tracks] == [1.0, 1.0, 1.0]
+    tracks = [[np.zeros(3)] * 6 + [w, 0] for w in (1.0, 4.0)]
+    calc.calculate_spectrum(tracks, timeStep=0.1, weights_normalize="max", Np_max=1, verbose=False)
+    assert tracks[0][6] == 1.0 and tracks[1][6] == 4.0   # Np_max cuts before normalising
+    assert calc.total_weight == 1.0
+
+
+class _Spec(Utilities):
+    def __init__(self, rad, Args):
+        self.Data = {"radiation": rad}
+        self.Args = Args
+
+
+def test_energy_integrals_exact_for_linear_integrands():
+    """get_energy_spectrum averages neighbouring theta samples and integrates
+    val*sin(theta_mid) with the trapezoid rule (synchrad/utils.py:81-86); a
+    spectrum built so that val_mid*sin(theta_mid) is linear in theta_mid is
+    integrated exactly.  get_energy then integrates over omega."""
+    theta = np.linspace(0.01, 0.2, 9)
+    th_mid = 0.5 * (theta[1:] + theta[:-1])
+    # choose val on the nodes so that midpoint averages equal c*th_mid/sin(th_mid)
+    target = th_mid / np.sin(th_mid)
+    val_nodes = np.zeros_like(theta)
+    val_nodes[0] = target[0]
+    for i in range(1, theta.size):
+        val_nodes[i] = 2 * target[i - 1] - val_nodes[i - 1]
+    omega = np.linspace(1.0, 3.0, 5)
+    phi = np.linspace(0, 2 * np.pi, 6, endpoint=False)
+    raw = omega[:, None, None] * val_nodes[None, :, None] * np.ones(phi.size)[None, None, :]
+    Args = {"mode": "far", "comp": "total", "omega": omega, "theta": theta, "phi": phi,
+            "dph": phi[1] - phi[0]}
+    s = _Spec({"total": raw[None]}, Args)
+    norm = alpha_fs / (4 * np.pi ** 2)
+    es = s.get_energy_spectrum()
+    want = norm * omega * (2 * np.pi) * 0.5 * (th_mid[-1] ** 2 - th_mid[0] ** 2)
+    assert np.allclose(es, want, rtol=1e-12)
+    assert s.get_energy() == pytest.approx(norm * 2 * np.pi * 0.5 * (th_mid[-1] ** 2 - th_mid[0] ** 2) * 4.0,
+                                           rel=1e-12)
+    assert np.allclose(s.get_full_spectrum(phot_num=True)[0], norm * raw[0] / omega[0])
+    assert np.allclose(s.get_full_spectrum(lambda0_um=2.0), norm * raw * J_in_um / 2.0)
+    Args["mode"] = "near"
+    Args["radius"] = theta
+    near = s.get_full_spectrum()
+    assert np.allclose(near, alpha_fs * np.pi / 4 / (2 * np.pi) ** 2 * raw)
+
+
+def test_workload_sizes_match_the_benchmark_definitions():
+    assert workloads.config1().evals == 256 * 32 * 32 * 1599
+    w2 = workloads.config2(n_particles=3)
+    assert w2.n_grid == 512 * 64 * 64 and w2.batch.n_intervals() == 3 * 7999
+    w4 = workloads.config4(n_particles=2)
+    assert w4.n_grid == 128 * 64 * 64 and w4.batch.n_intervals() == 2 * 3999
+    w5 = workloads.config5(n_particles=2)
+    assert w5.n_grid == 128 * 32 * 32 and w5.batch.n_intervals() == 2 * 3999
+    # fp32-rounded tracks, used as exact doubles
+    assert np.array_equal(w5.batch.x, w5.batch.x.astype(np.float32).astype(np.float64))
+
+
+def test_betatron_orbit_is_a_focusing_oscillation():
+    wl = workloads.config2(n_particles=5, n_steps=4000, grid=(8, 4, 4))
+    hb = wl.batch
+    kp = wl.extra["kp"]
+    x = hb.x.reshape(5, -1)
+    ux = hb.ux.reshape(5, -1)
+    g = np.sqrt(1 + hb.ux ** 2 + hb.uy ** 2 + hb.uz ** 2).reshape(5, -1)
+    # betatron wavenumber k_b = k_p / sqrt(2 gamma): count zero crossings of u_x
+    for p in range(5):
+        crossings = np.sum(np.diff(np.sign(ux[p])) != 0)
+        kb = kp / np.sqrt(2 * g[p].mean())
+        length = 4000 * hb.dt
+        assert abs(crossings - kb * length / np.pi) <= 2
+    # energy is conserved by a transverse force only to second order
+    assert np.all(np.abs(g[:, -1] / g[:, 0] - 1) < 1e-3)
+
+
+def test_config3_lengths_are_log_uniform():
+    wl = workloads.config3(scale=0.05, n_particles=400)
+    n = np.diff(wl.batch.offsets)
+    assert n.min() >= 200 and n.max() <= 20000
+    assert abs(np.mean(np.log(n)) - 0.5 * (np.log(200) + np.log(20000))) < 0.25
