@@ -55,6 +55,17 @@ enum {
 enum { SR_PREC_F64 = 0, SR_PREC_SPLIT = 1 };
 
 /*
+ * Sweep of the split path.  SR_SWEEP_DIRECT (what SR_SWEEP_AUTO selects) forms
+ * every phase in FP64.  SR_SWEEP_RECURRENCE, for grids that declare a constant
+ * omega_step, forms exp(i omega tau) in FP64/FP32 for one omega per thread and
+ * advances it to the thread's other omegas by complex multiplication; the
+ * |dphase| < pi filter is then applied as the equivalent threshold
+ * omega < pi/|tau_s - tau_{s-1}|.  The FP64 path always sweeps directly and
+ * filters exactly as the reference does.
+ */
+enum { SR_SWEEP_AUTO = 0, SR_SWEEP_DIRECT = 1, SR_SWEEP_RECURRENCE = 2 };
+
+/*
  * A batch of tracks packed CSR-style: particle p owns samples
  * [offsets[p], offsets[p+1]) of the x..uz arrays.  Replaces the per-particle
  * host loop of SynchRad.calculate_spectrum (synchrad/calc.py:257-267) and the
@@ -100,15 +111,18 @@ typedef struct sr_grid {
   const double *sin_phi;    /* device [n_phi] */
   const double *cos_phi;    /* device [n_phi] */
   double l_screen;          /* near: distance to the screen */
+  double omega_step;        /* omega[i+1] - omega[i] if constant (linear grids), else 0 */
 } sr_grid;
 
 typedef struct sr_opts {
   int32_t comp;             /* SR_COMP_* */
   int32_t precision;        /* SR_PREC_* */
-  int32_t omega_per_lane;   /* 0 = tuned, else 1 or 2 (split) / 1 (f64) */
+  int32_t omega_per_lane;   /* 0 = tuned; split direct: 1, 2, 4 (far) / 1, 2 (near);
+                               recurrence: 2, 4 (far) / 2 (near); f64: 1 */
   int32_t waves_per_block;  /* 0 = tuned, else 1, 2 or 4 */
   int32_t slices;           /* 0 = tuned, else 1..16 particle slices */
   int32_t persistent;       /* 0 = one block per tile; >0 = that many blocks loop over tiles */
+  int32_t sweep;            /* SR_SWEEP_* */
   const double *form_factor;/* device [n_omega]; far cartesian_complex only */
 } sr_opts;
 
@@ -119,8 +133,9 @@ const char *sr_last_error(void);
 int sr_num_outputs(int32_t mode, int32_t comp);
 
 /*
- * Workspace bytes sr_spectrum_batch needs for this request (0 when the
- * chosen plan uses no particle slices).
+ * Workspace bytes sr_spectrum_batch needs for this request: particle slices
+ * of one tile are accumulated in slices*n_out*nSnaps*n_grid doubles and then
+ * summed in slice order; 0 when the plan uses no slices.
  */
 int sr_workspace_bytes(const sr_tracks *tracks, const sr_grid *grid, const sr_opts *opts,
                        size_t *bytes);
@@ -135,6 +150,24 @@ int sr_workspace_bytes(const sr_tracks *tracks, const sr_grid *grid, const sr_op
 int sr_spectrum_batch(const sr_tracks *tracks, const sr_grid *grid, const sr_opts *opts,
                       double *const *outs, void *workspace, size_t workspace_bytes,
                       void *stream);
+
+/*
+ * Adjoint of the far-field spectrum (comp SR_COMP_TOTAL or SR_COMP_CARTESIAN,
+ * one snapshot).  No counterpart in the reference, which has no gradients.
+ * Given dS = dL/dS (1 or 3 device arrays laid out like the spectra) it ADDS
+ * dL/dx, dL/dy, dL/dz, dL/dux, dL/duy, dL/duz (device [n_points]) and dL/dw
+ * (device [n_particles]).  The |dphase| < pi mask of the forward is held
+ * fixed.  SR_PREC_F64 evaluates everything in FP64; SR_PREC_SPLIT forms
+ * phases in FP64 and sin/cos and the omega sums in FP32.  opts.slices sets the
+ * number of direction groups (0 = chosen); opts.waves_per_block the block size.
+ */
+int sr_far_adjoint_workspace_bytes(const sr_tracks *tracks, const sr_grid *grid, const sr_opts *opts,
+                                   size_t *bytes);
+int sr_far_adjoint_batch(const sr_tracks *tracks, const sr_grid *grid, const sr_opts *opts,
+                         const double *const *dS,
+                         double *gx, double *gy, double *gz,
+                         double *gux, double *guy, double *guz, double *gw,
+                         void *workspace, size_t workspace_bytes, void *stream);
 
 /*
  * Per-particle entry points with the argument order of the reference's

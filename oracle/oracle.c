@@ -31,6 +31,7 @@
 #include <math.h>
 #include <stdint.h>
 #include <stddef.h>
+#include <stdlib.h>
 #ifdef _OPENMP
 #include <omp.h>
 #endif
@@ -262,5 +263,187 @@ int oracle_near(int comp, double *const *spectra,
       }
     }
   }
+  return 0;
+}
+
+/*
+ * Far-field adjoint, one particle, FP64.  Given G = dL/dS on the grid (one
+ * array for `total`, three for `cartesian`, laid out like the spectra), adds
+ * dL/d(x, y, z, ux, uy, uz) for every sample of the particle and returns
+ * dL/dw in *gw.  S is the one snapshot record (nSnaps must be 1): S = w dt^2
+ * |F|^2 with F the sum over the active steps up to the record
+ * (kernel_farfield.cl:59-106); the |dphase| < pi mask is held fixed.
+ *
+ * Chain rule written per (grid point, step):
+ *   dL/dA_c   = H_c Re(conj(F_c) e^{i phi}),            H_c = 2 w dt^2 G_c
+ *   dL/dphi   = -sum_c H_c A_c Im(conj(F_c) e^{i phi}),  phi = omega (t - n.x)
+ *   dL/dx     = -omega n dL/dphi
+ *   A = c1 (n - m) - c2 a,  c2 = 1/(1 - n.m),  c1 = (n.a) c2^2
+ *   dL/dm     = 2 c1 c2 n ((n - m).gA) - c1 gA - c2^2 n (a.gA)
+ *   dL/da     = c2^2 n ((n - m).gA) - c2 gA
+ *   m = (b_s + b_{s+1})/2,  a = (b_{s+1} - b_s)/dt,  b = u/sqrt(1 + u.u)
+ *   dL/du     = (gb - b (b.gb)) sqrt(1 - b.b)
+ * gx..guz (nSteps each) are accumulated into, not overwritten.
+ */
+int oracle_far_adjoint(int comp, const double *const *G,
+                       const double *x, const double *y, const double *z,
+                       const double *ux, const double *uy, const double *uz,
+                       double wp, uint32_t itStart, uint32_t itEnd, uint32_t nSteps,
+                       const double *omega, const double *sinTheta, const double *cosTheta,
+                       const double *sinPhi, const double *cosPhi,
+                       uint32_t nOmega, uint32_t nTheta, uint32_t nPhi,
+                       double dt, uint32_t nSnaps, const uint32_t *itSnaps,
+                       double *gx, double *gy, double *gz,
+                       double *gux, double *guy, double *guz, double *gw, int nthreads) {
+  if (!(comp == OR_TOTAL || comp == OR_CARTESIAN)) return -1;
+  if (itEnd == 0 || nSteps == 0) return -2;
+  if (nSnaps != 1) return -4;
+  const int64_t nTotal = (int64_t)nTheta * nPhi * nOmega;
+  const uint32_t n = nSteps;
+  /* record step of the single snapshot (kernel_farfield.cl:53-57, 100) */
+  int64_t rec = -1;
+  {
+    const uint32_t iSnap = first_snap(itStart, nSnaps, itSnaps);
+    if (iSnap < nSnaps) {
+      const int64_t r = (int64_t)itSnaps[iSnap] - 2 - (int64_t)itStart;
+      if (r >= 0 && r <= (int64_t)itEnd - 2) rec = r;
+    }
+  }
+  if (rec < 0) return 0;
+  int64_t nAct = (int64_t)n - 1 < (int64_t)itEnd - 1 ? (int64_t)n - 1 : (int64_t)itEnd - 1;
+  if (nAct > rec + 1) nAct = rec + 1;
+  if (nAct <= 0) return 0;
+#ifdef _OPENMP
+  if (nthreads > 0) omp_set_num_threads(nthreads);
+  const int nth = omp_get_max_threads();
+#else
+  (void)nthreads;
+  const int nth = 1;
+#endif
+  /* per-thread accumulators: dL/dx (3n), dL/db (3n), dL/dw */
+  double *acc = (double *)calloc((size_t)nth * (6 * (size_t)n + 1), sizeof(double));
+  if (!acc) return -5;
+  const double dtInv = 1.0 / dt;
+
+#pragma omp parallel
+  {
+#ifdef _OPENMP
+    const int tno = omp_get_thread_num();
+#else
+    const int tno = 0;
+#endif
+    double *ax_ = acc + (size_t)tno * (6 * (size_t)n + 1);
+    double *ab_ = ax_ + 3 * (size_t)n;
+    double *aw_ = ab_ + 3 * (size_t)n;
+#pragma omp for schedule(static)
+    for (int64_t gti = 0; gti < nTotal; gti++) {
+      const uint32_t iPhi = (uint32_t)(gti / ((int64_t)nOmega * nTheta));
+      const uint32_t iTheta = (uint32_t)((gti - (int64_t)iPhi * nOmega * nTheta) / nOmega);
+      const uint32_t iOmega = (uint32_t)(gti - (int64_t)iPhi * nOmega * nTheta - (int64_t)iTheta * nOmega);
+      const double om = omega[iOmega];
+      const v3 nv = mk(sinTheta[iTheta] * cosPhi[iPhi], sinTheta[iTheta] * sinPhi[iPhi], cosTheta[iTheta]);
+      double Hc[3];
+      if (comp == OR_TOTAL) {
+        Hc[0] = Hc[1] = Hc[2] = 2.0 * wp * dt * dt * G[0][gti];
+      } else {
+        for (int c = 0; c < 3; c++) Hc[c] = 2.0 * wp * dt * dt * G[c][gti];
+      }
+      /* forward: F over the active steps up to the record */
+      double Fr[3] = {0, 0, 0}, Fi[3] = {0, 0, 0};
+      double prev = 0.0;
+      for (int64_t s = 0; s < nAct; s++) {
+        const double t = (double)(uint32_t)(itStart + (uint32_t)s) * dt;
+        const double ph = om * (t - dot3(mk(x[s], y[s], z[s]), nv));
+        const double d = fabs(ph - prev);
+        prev = ph;
+        if (!(d < M_PI)) continue;
+        v3 b0 = mk(ux[s], uy[s], uz[s]), b1 = mk(ux[s + 1], uy[s + 1], uz[s + 1]);
+        b0 = scl3(1.0 / sqrt(1.0 + dot3(b0, b0)), b0);
+        b1 = scl3(1.0 / sqrt(1.0 + dot3(b1, b1)), b1);
+        const v3 a = scl3(dtInv, sub3(b1, b0));
+        const v3 m = scl3(0.5, add3(b1, b0));
+        double c2 = 1.0 / (1.0 - dot3(m, nv));
+        double c1 = dot3(a, nv) * c2 * c2;
+        const v3 A = sub3(scl3(c1, sub3(nv, m)), scl3(c2, a));
+        const double cs = cos(ph), sn = sin(ph);
+        const double Av[3] = {A.x, A.y, A.z};
+        for (int c = 0; c < 3; c++) { Fr[c] += Av[c] * cs; Fi[c] += Av[c] * sn; }
+      }
+      if (comp == OR_TOTAL)
+        aw_[0] += dt * dt * G[0][gti] * (Fr[0] * Fr[0] + Fr[1] * Fr[1] + Fr[2] * Fr[2] +
+                                         Fi[0] * Fi[0] + Fi[1] * Fi[1] + Fi[2] * Fi[2]);
+      else
+        for (int c = 0; c < 3; c++) aw_[0] += dt * dt * G[c][gti] * (Fr[c] * Fr[c] + Fi[c] * Fi[c]);
+      /* backward */
+      prev = 0.0;
+      for (int64_t s = 0; s < nAct; s++) {
+        const double t = (double)(uint32_t)(itStart + (uint32_t)s) * dt;
+        const double ph = om * (t - dot3(mk(x[s], y[s], z[s]), nv));
+        const double d = fabs(ph - prev);
+        prev = ph;
+        if (!(d < M_PI)) continue;
+        v3 b0 = mk(ux[s], uy[s], uz[s]), b1 = mk(ux[s + 1], uy[s + 1], uz[s + 1]);
+        b0 = scl3(1.0 / sqrt(1.0 + dot3(b0, b0)), b0);
+        b1 = scl3(1.0 / sqrt(1.0 + dot3(b1, b1)), b1);
+        const v3 a = scl3(dtInv, sub3(b1, b0));
+        const v3 m = scl3(0.5, add3(b1, b0));
+        const double c2 = 1.0 / (1.0 - dot3(m, nv));
+        const double c1 = dot3(a, nv) * c2 * c2;
+        const v3 nm = sub3(nv, m);
+        const v3 A = sub3(scl3(c1, nm), scl3(c2, a));
+        const double cs = cos(ph), sn = sin(ph);
+        const double Av[3] = {A.x, A.y, A.z};
+        double gA[3], gph = 0.0;
+        for (int c = 0; c < 3; c++) {
+          const double qre = Fr[c] * cs + Fi[c] * sn;   /* Re(conj(F) e^{i ph}) */
+          const double qim = Fr[c] * sn - Fi[c] * cs;   /* Im(conj(F) e^{i ph}) */
+          gA[c] = Hc[c] * qre;
+          gph -= Hc[c] * Av[c] * qim;
+        }
+        const v3 gAv = mk(gA[0], gA[1], gA[2]);
+        /* x */
+        ax_[3 * s + 0] += -om * nv.x * gph;
+        ax_[3 * s + 1] += -om * nv.y * gph;
+        ax_[3 * s + 2] += -om * nv.z * gph;
+        /* m, a */
+        const double nmg = dot3(nm, gAv), ag = dot3(a, gAv);
+        const v3 gm = sub3(sub3(scl3(2.0 * c1 * c2 * nmg, nv), scl3(c1, gAv)), scl3(c2 * c2 * ag, nv));
+        const v3 ga = sub3(scl3(c2 * c2 * nmg, nv), scl3(c2, gAv));
+        /* b_s, b_{s+1} */
+        ab_[3 * s + 0] += 0.5 * gm.x - dtInv * ga.x;
+        ab_[3 * s + 1] += 0.5 * gm.y - dtInv * ga.y;
+        ab_[3 * s + 2] += 0.5 * gm.z - dtInv * ga.z;
+        ab_[3 * (s + 1) + 0] += 0.5 * gm.x + dtInv * ga.x;
+        ab_[3 * (s + 1) + 1] += 0.5 * gm.y + dtInv * ga.y;
+        ab_[3 * (s + 1) + 2] += 0.5 * gm.z + dtInv * ga.z;
+      }
+    }
+  }
+  /* reduce threads in order, then b -> u */
+  for (int tno = 0; tno < nth; tno++) {
+    const double *ax_ = acc + (size_t)tno * (6 * (size_t)n + 1);
+    const double *ab_ = ax_ + 3 * (size_t)n;
+    for (uint32_t s = 0; s < n; s++) {
+      gx[s] += ax_[3 * s];
+      gy[s] += ax_[3 * s + 1];
+      gz[s] += ax_[3 * s + 2];
+    }
+    *gw += ab_[3 * (size_t)n];
+  }
+  for (uint32_t s = 0; s < n; s++) {
+    double gb[3] = {0, 0, 0};
+    for (int tno = 0; tno < nth; tno++) {
+      const double *ab_ = acc + (size_t)tno * (6 * (size_t)n + 1) + 3 * (size_t)n;
+      for (int c = 0; c < 3; c++) gb[c] += ab_[3 * s + c];
+    }
+    const v3 u = mk(ux[s], uy[s], uz[s]);
+    const double gInv = 1.0 / sqrt(1.0 + dot3(u, u));
+    const v3 b = scl3(gInv, u);
+    const double bg = b.x * gb[0] + b.y * gb[1] + b.z * gb[2];
+    gux[s] += gInv * (gb[0] - b.x * bg);
+    guy[s] += gInv * (gb[1] - b.y * bg);
+    guz[s] += gInv * (gb[2] - b.z * bg);
+  }
+  free(acc);
   return 0;
 }

@@ -193,3 +193,60 @@ def spectrum(mode: str, comp: str, tracks: List[list], axes: Dict[str, np.ndarra
             raise RuntimeError(f"oracle returned {rc}")
     return {k: np.ascontiguousarray(o.swapaxes(-1, -3), dtype=np.double)
             for k, o in zip(keys, outs)}
+
+
+def far_adjoint(comp: str, tracks: List[list], axes: Dict[str, np.ndarray], timeStep: float,
+                G, it_range=None, threads: int = 0, library=None):
+    """dL/d(track) for L with dL/dS = G, far field, one snapshot (FP64).
+
+    ``G`` holds one array per output key of ``comp`` ('total' or 'cartesian') in
+    the host layout ``(1, nOmega, nTheta, nPhi)``.  Returns ``(grads, gw)``:
+    ``grads[p]`` is ``[gx, gy, gz, gux, guy, guz]`` for particle p and ``gw[p]``
+    its weight derivative.
+    """
+    L = library if library is not None else lib()
+    if not hasattr(L, "_adj_declared"):
+        L.oracle_far_adjoint.restype = ctypes.c_int
+        L.oracle_far_adjoint.argtypes = (
+            [ctypes.c_int, ctypes.POINTER(_dp)] + [_dp] * 6
+            + [ctypes.c_double, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32]
+            + [_dp] * 5 + [ctypes.c_uint32] * 3
+            + [ctypes.c_double, ctypes.c_uint32, _up] + [_dp] * 7 + [ctypes.c_int])
+        L._adj_declared = True
+    omega = axes["omega"].astype(np.double)
+    theta = axes["theta"].astype(np.double)
+    phi = axes["phi"].astype(np.double)
+    omega_dev = np.ascontiguousarray(np.double(2 * np.pi) * omega)
+    sinT, cosT = np.ascontiguousarray(np.sin(theta)), np.ascontiguousarray(np.cos(theta))
+    sinP, cosP = np.ascontiguousarray(np.sin(phi)), np.ascontiguousarray(np.cos(phi))
+    keys = COMP_KEYS[comp]
+    Gdev = [np.ascontiguousarray(np.asarray(G[k], dtype=np.double).swapaxes(-1, -3)) for k in keys]
+    g_ptrs = (_dp * len(Gdev))(*[_p(g) for g in Gdev])
+    global_snaps = snap_iterations(it_range, 1) if it_range is not None else None
+    grads, gws = [], []
+    for track in tracks:
+        if len(track) == 8:
+            x, y, z, ux, uy, uz, wp, it_start = track
+        else:
+            x, y, z, ux, uy, uz, wp = track
+            it_start = 0
+        arrs = [np.ascontiguousarray(np.asarray(a, dtype=np.double)) for a in (x, y, z, ux, uy, uz)]
+        n = arrs[0].size
+        out = [np.zeros(n) for _ in range(6)]
+        gw = np.zeros(1)
+        if it_range is None:
+            it_start, itEnd, snaps = 0, n, snap_iterations((0, n), 1)
+        else:
+            itEnd, snaps = int(it_range[-1]), global_snaps
+        if n > 0 and itEnd > 0:
+            rc = L.oracle_far_adjoint(COMP_CODES[comp], g_ptrs, *[_p(a) for a in arrs], float(wp),
+                                      int(np.uint32(it_start)), int(itEnd), int(n), _p(omega_dev),
+                                      _p(sinT), _p(cosT), _p(sinP), _p(cosP), omega.size, theta.size,
+                                      phi.size, float(timeStep), 1,
+                                      np.ascontiguousarray(snaps, dtype=np.uint32).ctypes.data_as(_up),
+                                      *[_p(o) for o in out], _p(gw), int(threads))
+            if rc != 0:
+                raise RuntimeError(f"oracle adjoint returned {rc}")
+        grads.append(out)
+        gws.append(gw[0])
+    return grads, np.array(gws)

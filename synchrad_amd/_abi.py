@@ -30,6 +30,10 @@ SR_COMP_SPHERIC_COMPLEX = 4
 SR_PREC_F64 = 0
 SR_PREC_SPLIT = 1
 
+SR_SWEEP_AUTO = 0
+SR_SWEEP_DIRECT = 1
+SR_SWEEP_RECURRENCE = 2
+
 MODES = {"far": SR_MODE_FAR, "near": SR_MODE_NEAR}
 COMPS = {
     "total": SR_COMP_TOTAL,
@@ -62,7 +66,7 @@ class SrGrid(ctypes.Structure):
     _fields_ = [
         ("mode", _i32), ("n_omega", _u32), ("n_theta", _u32), ("n_phi", _u32),
         ("omega", _vp), ("sin_theta", _vp), ("cos_theta", _vp), ("radius", _vp),
-        ("sin_phi", _vp), ("cos_phi", _vp), ("l_screen", _f64),
+        ("sin_phi", _vp), ("cos_phi", _vp), ("l_screen", _f64), ("omega_step", _f64),
     ]
 
 
@@ -70,14 +74,14 @@ class SrOpts(ctypes.Structure):
     _fields_ = [
         ("comp", _i32), ("precision", _i32), ("omega_per_lane", _i32),
         ("waves_per_block", _i32), ("slices", _i32), ("persistent", _i32),
-        ("form_factor", _vp),
+        ("sweep", _i32), ("form_factor", _vp),
     ]
 
 
 EXPORTS = (
     "sr_abi_version", "sr_last_error", "sr_num_outputs", "sr_workspace_bytes",
     "sr_spectrum_batch", "sr_far_total", "sr_far_cartesian", "sr_near_total",
-    "sr_near_cartesian",
+    "sr_near_cartesian", "sr_far_adjoint_workspace_bytes", "sr_far_adjoint_batch",
 )
 
 _lib = None
@@ -120,6 +124,12 @@ def load(path: str = LIB):
     L.sr_near_total.argtypes = [_vp] + track_args + [_vp] * 4 + [_f64] + tail
     L.sr_near_cartesian.restype = ctypes.c_int
     L.sr_near_cartesian.argtypes = [_vp] * 3 + track_args + [_vp] * 4 + [_f64] + tail
+    L.sr_far_adjoint_workspace_bytes.restype = ctypes.c_int
+    L.sr_far_adjoint_workspace_bytes.argtypes = L.sr_workspace_bytes.argtypes
+    L.sr_far_adjoint_batch.restype = ctypes.c_int
+    L.sr_far_adjoint_batch.argtypes = [ctypes.POINTER(SrTracks), ctypes.POINTER(SrGrid),
+                                       ctypes.POINTER(SrOpts), ctypes.POINTER(_vp)] + [_vp] * 7 + [
+                                       _vp, ctypes.c_size_t, _vp]
     v = L.sr_abi_version()
     if v != SR_ABI_VERSION:
         raise SynchradError(f"ABI version mismatch: library {v}, binding {SR_ABI_VERSION}")

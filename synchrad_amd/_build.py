@@ -12,7 +12,7 @@ import subprocess
 
 PKG_DIR = os.path.dirname(os.path.abspath(__file__))
 REPO_DIR = os.path.dirname(PKG_DIR)
-SRC = os.path.join(PKG_DIR, "csrc", "spectrum.hip")
+SRCS = [os.path.join(PKG_DIR, "csrc", f) for f in ("spectrum.hip", "adjoint.hip")]
 INCLUDE = os.path.join(REPO_DIR, "include")
 LIB = os.path.join(PKG_DIR, "libsynchrad_amd.so")
 ARCH = os.environ.get("SYNCHRAD_AMD_ARCH", "gfx950")
@@ -30,7 +30,7 @@ def command(out: str = LIB) -> list:
         hipcc(), f"--offload-arch={ARCH}", "-O3", "-std=c++17", "-fPIC", "-shared",
         "-ffp-contract=off",
         "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
-        "-I", INCLUDE, SRC, "-o", out,
+        "-I", INCLUDE, *SRCS, "-o", out,
     ]
 
 

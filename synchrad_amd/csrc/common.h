@@ -1,0 +1,108 @@
+// common.h -- constants, device helpers and host error reporting shared by the
+// spectrum and adjoint kernels of synchrad_amd.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <stdint.h>
+#include <math.h>
+
+namespace sr {
+
+constexpr int kWave = 64;
+constexpr int kMaxT = 256;        // threads per block = steps per LDS tile
+constexpr int kMaxSlices = 16;
+constexpr int kMaxOut = 6;
+
+constexpr double kPi = 3.141592653589793;
+constexpr double kInv2Pi = 0.15915494309189535;
+constexpr double k2PiHi = 6.283185307179586;
+constexpr double k2PiLo = 2.4492935982947064e-16;
+
+// Records a printf-style message for sr_last_error() and returns `code`.
+int fail(int code, const char *fmt, ...) __attribute__((format(printf, 2, 3)));
+const char *last_error();
+
+// ---------------------------------------------------------------------------
+// device helpers
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ int uni(int v) { return __builtin_amdgcn_readfirstlane(v); }
+__device__ __forceinline__ int64_t uni64(int64_t v) {
+  const uint32_t lo = (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)v);
+  const uint32_t hi = (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)((uint64_t)v >> 32));
+  return (int64_t)(((uint64_t)hi << 32) | lo);
+}
+__device__ __forceinline__ double unid(double v) {
+  return __longlong_as_double(uni64(__double_as_longlong(v)));
+}
+
+__device__ __forceinline__ void sincos_reduced(double ph, double *s, double *c) {
+  const double k = rint(ph * kInv2Pi);
+  double r = fma(-k, k2PiHi, ph);
+  r = fma(-k, k2PiLo, r);
+  sincos(r, s, c);
+}
+
+// sin/cos of 2*pi*rv from an FP64 count of revolutions: the fraction is taken
+// exactly in FP64, then the FP32 hardware units (input in revolutions) run.
+__device__ __forceinline__ void sincos_rev(double rv, float *s, float *c) {
+  const float f = (float)__builtin_amdgcn_fract(rv);
+  *s = __builtin_amdgcn_sinf(f);
+  *c = __builtin_amdgcn_cosf(f);
+}
+
+// 1/sqrt(1 + u.u) as the reference's rsqrt(1 + dot(u,u)), kernel_farfield.cl:77
+__device__ __forceinline__ double gamma_inv(double ux, double uy, double uz) {
+  return 1.0 / sqrt(1.0 + (ux * ux + uy * uy + uz * uz));
+}
+
+struct Particle {
+  int64_t off, n;
+  double w;
+  uint32_t its, ite;
+  const uint32_t *snaps;
+};
+
+template <class Params>
+__device__ __forceinline__ Particle load_particle(const Params &p, int64_t i) {
+  Particle r;
+  if (p.offsets) {
+    r.off = uni64(p.offsets[i]);
+    r.n = uni64(p.offsets[i + 1]) - r.off;
+    r.w = unid(p.weight[i]);
+    r.its = (uint32_t)uni((int)p.it_start[i]);
+    r.ite = (uint32_t)uni((int)p.it_end[i]);
+  } else {
+    r.off = 0;
+    r.n = p.single_n;
+    r.w = p.single_w;
+    r.its = p.single_its;
+    r.ite = p.single_ite;
+  }
+  r.snaps = p.it_snaps + i * (int64_t)p.snaps_stride;
+  return r;
+}
+
+// Snapshot schedule of one particle (kernel_farfield.cl:53-57, 100-106): the
+// record for slot iSnap happens after iteration r = itSnaps[iSnap] - 2 - itStart,
+// provided r lies in [0, itEnd-2] and after the previous record; a slot that
+// can never match stops all later records, as the equality test in the
+// reference never fires again.  Reading past the last slot means "no record".
+struct SnapCursor {
+  uint32_t slot;
+  int64_t next;  // -1: no further record
+  __device__ void advance(const Particle &pt, uint32_t n_snaps, int64_t last) {
+    next = -1;
+    if (slot >= n_snaps) return;
+    const int64_t r = (int64_t)(uint32_t)uni((int)pt.snaps[slot]) - 2 - (int64_t)pt.its;
+    if (r <= last || r > (int64_t)pt.ite - 2) return;
+    next = r;
+  }
+  __device__ void init(const Particle &pt, uint32_t n_snaps) {
+    slot = 0;
+    while (slot < n_snaps && !(pt.its < (uint32_t)uni((int)pt.snaps[slot]))) slot++;
+    advance(pt, n_snaps, -1);
+  }
+};
+
+}  // namespace sr

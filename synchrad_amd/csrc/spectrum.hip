@@ -13,27 +13,37 @@
 //
 // How it is laid out on gfx950:
 //   * A block owns one direction (far) or screen point (near) and a run of
-//     T*M consecutive omegas: each of the T = 64*waves threads holds M omegas
-//     (k-th omega of thread t is chunk*T*M + k*T + t, so every output store of
-//     a wave is 64 consecutive doubles).  A tile index also carries a particle
-//     slice q; slices of one tile are combined by a fixed-order reduction.
+//     T*M omegas: each of the T = 64*waves threads holds M omegas (the k-th
+//     omega of thread t is chunk*T*M + k*T + t, so every output access of a
+//     wave covers 64 consecutive doubles).  A tile index also carries a
+//     particle slice q; the slices of one tile are summed in slice order by a
+//     second kernel.
 //   * The block walks its particles (CSR) and their steps in tiles of T steps.
 //     Stage: thread t computes, in FP64, everything of step t0+t that does not
 //     depend on omega -- tau = t - n.x (far) or t + |P - x| (near) and the
 //     amplitude vector(s) -- and stores it in LDS.  Sweep: every thread reads
-//     the tile back (broadcast LDS reads) and for each of its omegas forms the
-//     phase omega*tau in FP64, applies the |dphase| < pi filter and
-//     accumulates.  The omega-independent work is therefore paid once per
-//     (point, step) and shared by the block's T*M omegas.
-//   * SR_PREC_F64: FP64 everywhere; the phase is reduced modulo 2 pi with a
-//     two-constant Cody-Waite step before the FP64 sincos, so the library
-//     routine only ever sees |r| <= pi.
-//   * SR_PREC_SPLIT: the phase and its reduction stay FP64, the reduced phase
-//     (in revolutions, |f| <= 1/2) feeds the hardware v_sin_f32/v_cos_f32,
-//     amplitudes are FP32 and the step sums are FP32 with a Kahan fold at the
-//     end of every swept segment (so the compensation costs O(1/T) per eval).
-//   * No atomics: every output element is owned by exactly one block (or one
-//     block per slice, reduced in slice order), so results are deterministic.
+//     the tile back with broadcast LDS reads and accumulates its omegas.  The
+//     omega-independent work is paid once per (point, step) and shared by the
+//     block's T*M omegas.
+//   * Sweeps:
+//       F64 direct    FP64 phase omega*tau, |dphase| < pi filter exactly as the
+//                     reference forms it, two-constant Cody-Waite reduction,
+//                     FP64 sincos, FP64 sums.
+//       split direct  same FP64 phase and filter; the reduced phase in
+//                     revolutions feeds v_sin_f32/v_cos_f32; FP32 amplitudes
+//                     and sums with a Kahan fold per swept segment.
+//       split rec     for omega grids with a constant step (linear grids): the
+//                     thread's M omegas differ by T*domega, so exp(i omega_k tau)
+//                     is advanced from k to k+1 by one complex multiply with
+//                     exp(i T domega tau), which the stage computes per step
+//                     (FP64 phase, FP64 reduction, FP32 sin/cos).  The filter
+//                     becomes the per-step threshold omega < pi/|tau_s - tau_{s-1}|
+//                     (the same inequality with omega factored out), and whole
+//                     steps are skipped or taken unmasked when the block's omega
+//                     range lies entirely on one side of it.
+//   * Outputs are read-modify-written once per particle record by the one
+//     thread that owns them: no atomics, deterministic results, and the
+//     particle order of the reference's sequential `spectrum +=` is kept.
 //
 // All code here is compiled with -ffp-contract=off: the FP64 stage reproduces
 // the reference's expression order term by term, and fused multiply-adds are
@@ -50,22 +60,12 @@
 #include <type_traits>
 
 #include "synchrad_amd.h"
+#include "common.h"
 
-namespace {
-
-constexpr int kWave = 64;
-constexpr int kMaxT = 256;        // threads per block = steps per LDS tile
-constexpr int kMaxSlices = 16;
-constexpr int kMaxOut = 6;
-
-constexpr double kPi = 3.141592653589793;
-constexpr double kInv2Pi = 0.15915494309189535;
-constexpr double k2PiHi = 6.283185307179586;
-constexpr double k2PiLo = 2.4492935982947064e-16;
+namespace sr {
 
 thread_local char g_err[512] = "";
 
-int fail(int code, const char *fmt, ...) __attribute__((format(printf, 2, 3)));
 int fail(int code, const char *fmt, ...) {
   va_list ap;
   va_start(ap, fmt);
@@ -73,6 +73,16 @@ int fail(int code, const char *fmt, ...) {
   va_end(ap);
   return code;
 }
+
+const char *last_error() { return g_err; }
+
+}  // namespace sr
+
+namespace {
+
+using namespace sr;
+
+enum { SWEEP_DIRECT = 0, SWEEP_REC = 1 };
 
 struct KParams {
   // tracks (CSR); offsets == nullptr selects the single-particle form
@@ -90,12 +100,13 @@ struct KParams {
   uint32_t n_omega, n_theta, n_phi;
   const double *omega, *sin_t, *cos_t, *radius, *sin_p, *cos_p;
   double l_screen;
+  double omega_step;        // rec sweep: omega[i+1] - omega[i]
   const double *form_factor;
   int64_t n_grid;
-  // outputs
+  // outputs (slices > 1: scratch[q][c] replaces out[c])
   int32_t n_out;
   double *out[kMaxOut];
-  double *scratch;  // slices > 1: [slices][n_out][n_grid]
+  double *scratch;
   // tiling
   uint32_t n_chunks;
   uint32_t n_dir;
@@ -105,85 +116,56 @@ struct KParams {
 };
 
 // ---------------------------------------------------------------------------
-// device helpers
-// ---------------------------------------------------------------------------
-__device__ __forceinline__ void sincos_reduced(double ph, double *s, double *c) {
-  const double k = rint(ph * kInv2Pi);
-  double r = fma(-k, k2PiHi, ph);
-  r = fma(-k, k2PiLo, r);
-  sincos(r, s, c);
-}
-
-// 1/sqrt(1 + u.u) as the reference's rsqrt(1 + dot(u,u)), kernel_farfield.cl:77
-__device__ __forceinline__ double gamma_inv(double ux, double uy, double uz) {
-  return 1.0 / sqrt(1.0 + (ux * ux + uy * uy + uz * uz));
-}
-
-struct Particle {
-  int64_t off, n;
-  double w;
-  uint32_t its, ite;
-  const uint32_t *snaps;
-};
-
-__device__ __forceinline__ Particle load_particle(const KParams &p, int64_t i) {
-  Particle r;
-  if (p.offsets) {
-    r.off = p.offsets[i];
-    r.n = p.offsets[i + 1] - r.off;
-    r.w = p.weight[i];
-    r.its = p.it_start[i];
-    r.ite = p.it_end[i];
-  } else {
-    r.off = 0;
-    r.n = p.single_n;
-    r.w = p.single_w;
-    r.its = p.single_its;
-    r.ite = p.single_ite;
-  }
-  r.snaps = p.it_snaps + i * (int64_t)p.snaps_stride;
-  return r;
-}
-
-// Snapshot schedule of one particle (kernel_farfield.cl:53-57, 100-106): the
-// record for slot iSnap happens after iteration r = itSnaps[iSnap] - 2 - itStart,
-// provided r lies in [0, itEnd-2] and after the previous record; a slot that
-// can never match stops all later records, as the equality test in the
-// reference never fires again.  Reading past the last slot means "no record".
-struct SnapCursor {
-  uint32_t slot;
-  int64_t next;  // -1: no further record
-  __device__ void advance(const Particle &pt, uint32_t n_snaps, int64_t last) {
-    next = -1;
-    if (slot >= n_snaps) return;
-    const int64_t r = (int64_t)pt.snaps[slot] - 2 - (int64_t)pt.its;
-    if (r <= last || r > (int64_t)pt.ite - 2) return;
-    next = r;
-  }
-  __device__ void init(const Particle &pt, uint32_t n_snaps) {
-    slot = 0;
-    while (slot < n_snaps && !(pt.its < pt.snaps[slot])) slot++;
-    advance(pt, n_snaps, -1);
-  }
-};
-
-// ---------------------------------------------------------------------------
 // the tile kernel
 // ---------------------------------------------------------------------------
-template <int MODE, int PREC, int M>
-__global__ void __launch_bounds__(kMaxT) k_spectrum(const KParams p) {
+// Dynamic LDS of the split sweeps: the Kahan sum S and compensation C of every
+// thread's NC*M accumulators, [2][NC*M][T] floats; a thread only ever touches
+// its own column, so no barrier guards them.
+template <int MODE, int M>
+constexpr size_t kahan_lds_bytes(int T) {
+  return (size_t)2 * (MODE == SR_MODE_FAR ? 6 : 12) * M * T * sizeof(float);
+}
+
+// Register budget per variant, as the minimum number of waves each SIMD must
+// be able to hold: the split sweeps keep their Kahan state in LDS, so their
+// registers hold the partial sums of M omegas plus one staged step group.
+#ifndef SR_REC_WAVES
+#define SR_REC_WAVES 2
+#endif
+#ifndef SR_REC_GROUP
+#define SR_REC_GROUP 4
+#endif
+template <int PREC, int SWEEP, int M>
+struct WavesPerEU {
+  static constexpr int value = (PREC == SR_PREC_F64) ? 3 : (SWEEP == SWEEP_REC ? SR_REC_WAVES : (M >= 4 ? 2 : 3));
+};
+
+template <int MODE, int PREC, int SWEEP, int M>
+__global__ void __launch_bounds__(kMaxT)
+__attribute__((amdgpu_waves_per_eu(WavesPerEU<PREC, SWEEP, M>::value)))
+k_spectrum(const KParams p) {
+  static_assert(!(PREC == SR_PREC_F64 && SWEEP == SWEEP_REC), "recurrence sweep is split-only");
+  static_assert(PREC == SR_PREC_SPLIT || M == 1, "the FP64 sweep holds one omega per thread");
   using Amp = typename std::conditional<PREC == SR_PREC_SPLIT, float, double>::type;
+  constexpr bool kSplit = (PREC == SR_PREC_SPLIT);
+  constexpr bool kRec = (SWEEP == SWEEP_REC);
   constexpr int NA = (MODE == SR_MODE_FAR) ? 3 : 6;   // amplitude words per step
   constexpr int NC = 2 * NA;                           // real accumulators per omega
 
-  __shared__ double s_tau[kMaxT];
-  __shared__ Amp s_amp[NA][kMaxT];
+  __shared__ __align__(16) double s_tau[kMaxT];
+  __shared__ __align__(16) Amp s_amp[NA][kMaxT];
+  __shared__ __align__(16) double s_thr[kRec ? kMaxT : 1];
+  __shared__ __align__(16) float s_rot[2][kRec ? kMaxT : 1];
+  extern __shared__ __align__(16) float s_dyn[];
 
   const int T = blockDim.x;
   const int tid = threadIdx.x;
+  float *const s_ks = s_dyn + tid;                       // S[c*M+k] at s_ks[(c*M+k)*T]
+  float *const s_kc = s_dyn + (size_t)NC * M * T + tid;  // C likewise
   const double dt = p.dt;
   const double dtInv = 1.0 / dt;
   const bool spheric = (p.comp == SR_COMP_SPHERIC || p.comp == SR_COMP_SPHERIC_COMPLEX);
+  const double rot_rev = (double)T * p.omega_step * kInv2Pi;   // revolutions of the k -> k+1 step
 
   for (int64_t tile = blockIdx.x; tile < p.n_tiles; tile += gridDim.x) {
     const uint32_t chunk = (uint32_t)(tile % p.n_chunks);
@@ -212,20 +194,35 @@ __global__ void __launch_bounds__(kMaxT) k_spectrum(const KParams p) {
       gz = p.l_screen;
     }
 
-    int64_t wi[M];
+    int wi[M];
     bool valid[M];
     double om[M];
 #pragma unroll
     for (int k = 0; k < M; k++) {
-      wi[k] = (int64_t)chunk * T * M + (int64_t)k * T + tid;
-      valid[k] = wi[k] < (int64_t)p.n_omega;
+      wi[k] = (int)(chunk * T * M + k * T + tid);
+      valid[k] = wi[k] < (int)p.n_omega;
       om[k] = valid[k] ? p.omega[wi[k]] : 0.0;
     }
-    double acc[kMaxOut][M];
+    double om_rev[M];
+#pragma unroll
+    for (int k = 0; k < M; k++) om_rev[k] = om[k] * kInv2Pi;
+    // rec sweep: the thread's base frequency in revolutions, the block's omega range
+    const double om0_rev = om_rev[0];
+    double w_lo = 0.0, w_hi = 0.0;
+    if (kRec) {
+      const int64_t first = (int64_t)chunk * T * M;
+      const int64_t last = min((int64_t)(chunk + 1) * T * M, (int64_t)p.n_omega) - 1;
+      const double a = p.omega[first], b = p.omega[last];
+      w_lo = fmin(a, b);
+      w_hi = fmax(a, b);
+    }
+    double *dst[kMaxOut];
 #pragma unroll
     for (int c = 0; c < kMaxOut; c++)
-#pragma unroll
-      for (int k = 0; k < M; k++) acc[c][k] = 0.0;
+      dst[c] = (c < p.n_out)
+                   ? (p.slices > 1 ? p.scratch + ((int64_t)q * p.n_out + c) * p.n_grid * p.n_snaps
+                                   : p.out[c])
+                   : nullptr;
 
     const int64_t p_lo = p.slice_lo[q], p_hi = p.slice_lo[q + 1];
     for (int64_t ip = p_lo; ip < p_hi; ip++) {
@@ -238,28 +235,42 @@ __global__ void __launch_bounds__(kMaxT) k_spectrum(const KParams p) {
       SnapCursor cur;
       cur.init(pt, p.n_snaps);
 
+      // F64: running sums fs; split: segment partials ps (registers), Kahan
+      // sum and compensation in LDS
       double prev[M];
-      // F64: running sums; SPLIT: tile partial P, Kahan sum S and compensation C
       double fs[M][NC];
-      float ps[M][NC], ss[M][NC], cs_[M][NC];
+      float ps[M][NC];
 #pragma unroll
       for (int k = 0; k < M; k++) {
         prev[k] = 0.0;
 #pragma unroll
         for (int c = 0; c < NC; c++) {
           fs[k][c] = 0.0;
-          ps[k][c] = 0.f; ss[k][c] = 0.f; cs_[k][c] = 0.f;
+          ps[k][c] = 0.f;
+        }
+      }
+      if (kSplit) {
+#pragma unroll
+        for (int i = 0; i < NC * M; i++) {
+          s_ks[i * T] = 0.f;
+          s_kc[i * T] = 0.f;
         }
       }
 
-      // adds this particle's snapshot value into the block's outputs
+      // adds this particle's snapshot value into the outputs the thread owns.
+      // Records are rare; the split sweeps walk their omegas one at a time
+      // (not unrolled, everything re-derived from k) so that the record code
+      // does not set the kernel's register budget.
       auto emit = [&](uint32_t slot) {
-#pragma unroll
+#pragma unroll 1
         for (int k = 0; k < M; k++) {
+          const int wk = (int)(chunk * T * M + k * T + tid);
+          if (wk >= (int)p.n_omega) continue;
+          const double omk = kSplit ? p.omega[wk] : om[0];
           double F[NC];
 #pragma unroll
           for (int c = 0; c < NC; c++)
-            F[c] = (PREC == SR_PREC_SPLIT) ? ((double)ss[k][c] - (double)cs_[k][c]) : fs[k][c];
+            F[c] = kSplit ? ((double)s_ks[(c * M + k) * T] - (double)s_kc[(c * M + k) * T]) : fs[0][c];
           double re[3], im[3];
           if (MODE == SR_MODE_FAR) {
 #pragma unroll
@@ -268,8 +279,8 @@ __global__ void __launch_bounds__(kMaxT) k_spectrum(const KParams p) {
             // F = G2 + i omega G1 ; words: G1re 0..2, G1im 3..5, G2re 6..8, G2im 9..11
 #pragma unroll
             for (int c = 0; c < 3; c++) {
-              re[c] = F[6 + c] - om[k] * F[3 + c];
-              im[c] = F[9 + c] + om[k] * F[c];
+              re[c] = F[6 + c] - omk * F[3 + c];
+              im[c] = F[9 + c] + omk * F[c];
             }
           }
           double v[kMaxOut];
@@ -284,8 +295,8 @@ __global__ void __launch_bounds__(kMaxT) k_spectrum(const KParams p) {
               for (int c = 0; c < 3; c++) v[c] = wpdt2 * (re[c] * re[c] + im[c] * im[c]);
               break;
             default: {
-              const double ff = (MODE == SR_MODE_FAR && p.comp == SR_COMP_CARTESIAN_COMPLEX && valid[k])
-                                    ? p.form_factor[wi[k]] : 1.0;
+              const double ff = (MODE == SR_MODE_FAR && p.comp == SR_COMP_CARTESIAN_COMPLEX)
+                                    ? p.form_factor[wk] : 1.0;
 #pragma unroll
               for (int c = 0; c < 3; c++) {
                 v[2 * c] = wpdt * (ff * re[c]);
@@ -293,22 +304,89 @@ __global__ void __launch_bounds__(kMaxT) k_spectrum(const KParams p) {
               }
             }
           }
-          if (p.n_snaps == 1) {
+          const int64_t g = (int64_t)slot * p.n_grid + (int64_t)dir * p.n_omega + wk;
 #pragma unroll
-            for (int c = 0; c < kMaxOut; c++)
-              if (c < p.n_out) acc[c][k] += v[c];
-          } else if (valid[k]) {
-            const int64_t g = (int64_t)slot * p.n_grid + (int64_t)dir * p.n_omega + wi[k];
+          for (int c = 0; c < kMaxOut; c++)
+            if (c < p.n_out) dst[c][g] += v[c];
+        }
+      };
+
+      // one step of the recurrence sweep: the base phase in FP64, then each of
+      // the thread's omegas by rotation.  MASKED: the step is skipped when the
+      // whole block is above its threshold and each omega is masked otherwise;
+      // the unmasked form runs for tiles where no step cuts the block's range.
+      auto rec_step = [&](auto masked, double tau, double thr, float er, float ei, const float *a) {
+        constexpr bool kMasked = decltype(masked)::value;
+        if (kMasked && thr <= w_lo) return;        // every omega of the block filtered out
+        float sn, cn;
+        sincos_rev(om0_rev * tau, &sn, &cn);
 #pragma unroll
-            for (int c = 0; c < kMaxOut; c++)
-              if (c < p.n_out) p.out[c][g] += v[c];
+        for (int k = 0; k < M; k++) {
+          float cm = cn, sm = sn;
+          if (kMasked) {
+            const bool act = om[k] < thr;
+            cm = act ? cn : 0.f;
+            sm = act ? sn : 0.f;
           }
+#pragma unroll
+          for (int c = 0; c < NA; c++) {
+            const int w = 6 * (c / 3) + (c % 3);
+            ps[k][w] = __builtin_fmaf(a[c], cm, ps[k][w]);
+            ps[k][w + 3] = __builtin_fmaf(a[c], sm, ps[k][w + 3]);
+          }
+          if (k + 1 < M) {
+            const float cn2 = __builtin_fmaf(cn, er, -sn * ei);
+            sn = __builtin_fmaf(cn, ei, sn * er);
+            cn = cn2;
+          }
+        }
+      };
+      // steps [j, jend) of the staged tile, single steps up to a multiple of 4,
+      // then groups of 4 fetched with 16-byte LDS reads, then the tail
+      auto rec_sweep = [&](auto masked, int &j, int jend) {
+        const int jg = (SR_REC_GROUP == 4) ? min(jend, (j + 3) & ~3) : jend;
+        for (; j < jg; j++) {
+          float a[NA];
+#pragma unroll
+          for (int c = 0; c < NA; c++) a[c] = (float)s_amp[c][j];
+          rec_step(masked, s_tau[j], s_thr[j], s_rot[0][j], s_rot[1][j], a);
+        }
+        for (; j + 4 <= jend; j += 4) {
+          const double2 t01 = *(const double2 *)&s_tau[j];
+          const double2 t23 = *(const double2 *)&s_tau[j + 2];
+          const double2 h01 = *(const double2 *)&s_thr[j];
+          const double2 h23 = *(const double2 *)&s_thr[j + 2];
+          const float4 er = *(const float4 *)&s_rot[0][j];
+          const float4 ei = *(const float4 *)&s_rot[1][j];
+          float4 av[NA];
+#pragma unroll
+          for (int c = 0; c < NA; c++) av[c] = *(const float4 *)&s_amp[c][j];
+          float a[NA];
+#pragma unroll
+          for (int c = 0; c < NA; c++) a[c] = av[c].x;
+          rec_step(masked, t01.x, h01.x, er.x, ei.x, a);
+#pragma unroll
+          for (int c = 0; c < NA; c++) a[c] = av[c].y;
+          rec_step(masked, t01.y, h01.y, er.y, ei.y, a);
+#pragma unroll
+          for (int c = 0; c < NA; c++) a[c] = av[c].z;
+          rec_step(masked, t23.x, h23.x, er.z, ei.z, a);
+#pragma unroll
+          for (int c = 0; c < NA; c++) a[c] = av[c].w;
+          rec_step(masked, t23.y, h23.y, er.w, ei.w, a);
+        }
+        for (; j < jend; j++) {
+          float a[NA];
+#pragma unroll
+          for (int c = 0; c < NA; c++) a[c] = (float)s_amp[c][j];
+          rec_step(masked, s_tau[j], s_thr[j], s_rot[0][j], s_rot[1][j], a);
         }
       };
 
       for (int64_t t0 = 0; t0 < n_act; t0 += T) {
-        const int cnt = (int)min((int64_t)T, n_act - t0);
+        const int cnt = uni((int)min((int64_t)T, n_act - t0));
         __syncthreads();  // previous tile fully consumed
+        int step_on = 1;
         if (tid < cnt) {
           // ---- stage: omega-independent part of step s = t0 + tid, in FP64 ----
           const int64_t s = t0 + tid;
@@ -318,9 +396,14 @@ __global__ void __launch_bounds__(kMaxT) k_spectrum(const KParams p) {
           const double u0x = p.ux[gs], u0y = p.uy[gs], u0z = p.uz[gs];
           const double g0 = gamma_inv(u0x, u0y, u0z);
           const double b0x = u0x * g0, b0y = u0y * g0, b0z = u0z * g0;
+          double tau, tau_prev = 0.0;
           if (MODE == SR_MODE_FAR) {
             // kernel_farfield.cl:65-94
-            s_tau[tid] = time - (xs * gx + ys * gy + zs * gz);
+            tau = time - (xs * gx + ys * gy + zs * gz);
+            if (kRec && s > 0) {
+              const double tp = (double)(uint32_t)(pt.its + (uint32_t)(s - 1)) * dt;
+              tau_prev = tp - (p.x[gs - 1] * gx + p.y[gs - 1] * gy + p.z[gs - 1] * gz);
+            }
             const double u1x = p.ux[gs + 1], u1y = p.uy[gs + 1], u1z = p.uz[gs + 1];
             const double g1 = gamma_inv(u1x, u1y, u1z);
             const double b1x = u1x * g1, b1y = u1y * g1, b1z = u1z * g1;
@@ -347,7 +430,12 @@ __global__ void __launch_bounds__(kMaxT) k_spectrum(const KParams p) {
             // kernel_nearfield.cl:67-88; omega of c1 is applied at the record
             const double rx = gx - xs, ry = gy - ys, rz = gz - zs;
             const double R = sqrt(rx * rx + ry * ry + rz * rz);
-            s_tau[tid] = time + R;
+            tau = time + R;
+            if (kRec && s > 0) {
+              const double tp = (double)(uint32_t)(pt.its + (uint32_t)(s - 1)) * dt;
+              const double qx = gx - p.x[gs - 1], qy = gy - p.y[gs - 1], qz = gz - p.z[gs - 1];
+              tau_prev = tp + sqrt(qx * qx + qy * qy + qz * qz);
+            }
             const double rInv = 1.0 / R;
             const double nx = rInv * rx, ny = rInv * ry, nz = rInv * rz;
             const double r2 = rInv * rInv;
@@ -358,8 +446,21 @@ __global__ void __launch_bounds__(kMaxT) k_spectrum(const KParams p) {
             s_amp[4][tid] = (Amp)(r2 * ny);
             s_amp[5][tid] = (Amp)(r2 * nz);
           }
+          s_tau[tid] = tau;
+          if (kRec) {
+            // |omega tau_s - omega tau_{s-1}| < pi  <=>  omega < pi/|tau_s - tau_{s-1}|
+            // (phase_{-1} = 0 at the first step); 1/0 gives +inf: always active
+            const double thr = kPi / fabs(tau - tau_prev);
+            s_thr[tid] = thr;
+            step_on = thr > w_hi;
+            float sr, cr;
+            sincos_rev(rot_rev * tau, &sr, &cr);
+            s_rot[0][tid] = cr;
+            s_rot[1][tid] = sr;
+          }
         }
-        __syncthreads();
+        // rec: the tile needs no masking when every step keeps the whole block
+        const bool tile_on = kRec ? (__syncthreads_and(step_on) != 0) : (__syncthreads(), false);
 
         // ---- sweep, split at snapshot records falling inside the tile ----
         int j = 0;
@@ -367,65 +468,68 @@ __global__ void __launch_bounds__(kMaxT) k_spectrum(const KParams p) {
           const bool rec_here = (cur.next >= 0) && (cur.next < t0 + cnt);
           // records are strictly increasing, so cur.next >= t0 + j; the max()
           // only guarantees forward progress
-          const int jend = rec_here ? max(j + 1, (int)(cur.next - t0) + 1) : cnt;
-          for (; j < jend; j++) {
-            const double tau = s_tau[j];
-            Amp a[NA];
+          const int jend = uni(rec_here ? max(j + 1, (int)(cur.next - t0) + 1) : cnt);
+          if (kRec) {
+            if (tile_on)
+              rec_sweep(std::false_type(), j, jend);
+            else
+              rec_sweep(std::true_type(), j, jend);
+          } else {
+#pragma unroll 2
+            for (; j < jend; j++) {
+              const double tau = s_tau[j];
+              Amp a[NA];
 #pragma unroll
-            for (int c = 0; c < NA; c++) a[c] = s_amp[c][j];
+              for (int c = 0; c < NA; c++) a[c] = s_amp[c][j];
 #pragma unroll
-            for (int k = 0; k < M; k++) {
-              const double ph = om[k] * tau;
-              const double d = ph - prev[k];
-              prev[k] = ph;
-              const bool act = valid[k] && (fabs(d) < kPi);
-              if (act) {
-                if (PREC == SR_PREC_SPLIT) {
-                  const double rv = ph * kInv2Pi;
-                  const float f = (float)(rv - rint(rv));
-                  const float sn = __builtin_amdgcn_sinf(f);
-                  const float cn = __builtin_amdgcn_cosf(f);
-                  // word layout shared with the FP64 path: per amplitude vector
-                  // v (far: A; near: v1 then v2) [v cos | v sin], 3 words each
+              for (int k = 0; k < M; k++) {
+                // FP64: phase in radians and the reference's exact |dphase| < pi
+                // test.  Split: phase in revolutions (omega/2pi precomputed), so
+                // the test reads |dphase| < 1/2 and the fraction feeds the FP32
+                // units directly; decisions can differ from the radian test
+                // only within a few ulp of the boundary.
+                const double ph = (kSplit ? om_rev[k] : om[k]) * tau;
+                const double d = ph - prev[k];
+                prev[k] = ph;
+                const bool act = valid[k] && (fabs(d) < (kSplit ? 0.5 : kPi));
+                if (act) {
+                  if (kSplit) {
+                    float sn, cn;
+                    sincos_rev(ph, &sn, &cn);
+                    // word layout shared with the FP64 path: per amplitude vector
+                    // v (far: A; near: v1 then v2) [v cos | v sin], 3 words each
 #pragma unroll
-                  for (int c = 0; c < NA; c++) {
-                    const int w = 6 * (c / 3) + (c % 3);
-                    ps[k][w] = __builtin_fmaf((float)a[c], cn, ps[k][w]);
-                    ps[k][w + 3] = __builtin_fmaf((float)a[c], sn, ps[k][w + 3]);
-                  }
-                } else {
-                  double sn, cn;
-                  sincos_reduced(ph, &sn, &cn);
-                  if (MODE == SR_MODE_FAR) {
-#pragma unroll
-                    for (int c = 0; c < 3; c++) {
-                      fs[k][c] = fma((double)a[c], cn, fs[k][c]);
-                      fs[k][3 + c] = fma((double)a[c], sn, fs[k][3 + c]);
+                    for (int c = 0; c < NA; c++) {
+                      const int w = 6 * (c / 3) + (c % 3);
+                      ps[k][w] = __builtin_fmaf((float)a[c], cn, ps[k][w]);
+                      ps[k][w + 3] = __builtin_fmaf((float)a[c], sn, ps[k][w + 3]);
                     }
                   } else {
-                    // G1 += v1 e^{i ph}, G2 += v2 e^{i ph}
+                    double sn, cn;
+                    sincos_reduced(ph, &sn, &cn);
 #pragma unroll
-                    for (int c = 0; c < 3; c++) {
-                      fs[k][c] = fma((double)a[c], cn, fs[k][c]);
-                      fs[k][3 + c] = fma((double)a[c], sn, fs[k][3 + c]);
-                      fs[k][6 + c] = fma((double)a[3 + c], cn, fs[k][6 + c]);
-                      fs[k][9 + c] = fma((double)a[3 + c], sn, fs[k][9 + c]);
+                    for (int c = 0; c < NA; c++) {
+                      const int w = 6 * (c / 3) + (c % 3);
+                      fs[k][w] = fma((double)a[c], cn, fs[k][w]);
+                      fs[k][w + 3] = fma((double)a[c], sn, fs[k][w + 3]);
                     }
                   }
                 }
               }
             }
           }
-          if (PREC == SR_PREC_SPLIT) {
-            // Kahan fold of the segment's partial sums
+          if (kSplit) {
+            // Kahan fold of the segment's partial sums into the LDS S, C
 #pragma unroll
             for (int k = 0; k < M; k++)
 #pragma unroll
               for (int c = 0; c < NC; c++) {
-                const float yv = ps[k][c] - cs_[k][c];
-                const float tv = ss[k][c] + yv;
-                cs_[k][c] = (tv - ss[k][c]) - yv;
-                ss[k][c] = tv;
+                float &S = s_ks[(c * M + k) * T];
+                float &C = s_kc[(c * M + k) * T];
+                const float yv = ps[k][c] - C;
+                const float tv = S + yv;
+                C = (tv - S) - yv;
+                S = tv;
                 ps[k][c] = 0.f;
               }
           }
@@ -445,35 +549,26 @@ __global__ void __launch_bounds__(kMaxT) k_spectrum(const KParams p) {
         cur.advance(pt, p.n_snaps, last);
       }
     }
-
-    if (p.n_snaps == 1) {
-#pragma unroll
-      for (int k = 0; k < M; k++) {
-        if (!valid[k]) continue;
-        const int64_t g = (int64_t)dir * p.n_omega + wi[k];
-#pragma unroll
-        for (int c = 0; c < kMaxOut; c++) {
-          if (c >= p.n_out) break;
-          if (p.slices > 1)
-            p.scratch[((int64_t)q * p.n_out + c) * p.n_grid + g] = acc[c][k];
-          else
-            p.out[c][g] += acc[c][k];
-        }
-      }
-    }
   }
 }
+
 
 // out[c][i] += sum_q scratch[q][c][i], q in order
 __global__ void k_reduce_slices(const KParams p) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  const int64_t total = (int64_t)p.n_out * p.n_grid;
+  const int64_t total = (int64_t)p.n_out * p.n_grid * p.n_snaps;
   if (i >= total) return;
-  const int c = (int)(i / p.n_grid);
-  const int64_t g = i - (int64_t)c * p.n_grid;
+  const int64_t per = (int64_t)p.n_grid * p.n_snaps;
+  const int c = (int)(i / per);
+  const int64_t g = i - (int64_t)c * per;
   double s = 0.0;
-  for (int q = 0; q < p.slices; q++) s += p.scratch[((int64_t)q * p.n_out + c) * p.n_grid + g];
+  for (int q = 0; q < p.slices; q++) s += p.scratch[((int64_t)q * p.n_out + c) * per + g];
   p.out[c][g] += s;
+}
+
+__global__ void k_fill_zero(double *buf, int64_t n) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) buf[i] = 0.0;
 }
 
 // ---------------------------------------------------------------------------
@@ -497,70 +592,93 @@ int outputs_for(int32_t mode, int32_t comp) {
 }
 
 struct Plan {
+  int sweep;     // SWEEP_*
   int waves;     // waves per block
   int M;         // omegas per thread
   int slices;
   int64_t blocks;
 };
 
-// Tile shape: prefer the largest block whose omega run is mostly used,
-// then enough particle slices to put ~8 waves on each of the 1024 SIMDs.
+// omegas per thread each variant is built for (the near-field recurrence
+// keeps twice the far field's accumulators, so it stops at 2)
+int allowed_m(int mode, int prec, int sweep, int m) {
+  if (prec == SR_PREC_F64) return m == 1;
+  if (sweep == SWEEP_REC) return mode == SR_MODE_FAR ? (m == 2 || m == 4) : m == 2;
+  return m == 1 || m == 2 || (mode == SR_MODE_FAR && m == 4);
+}
+
+// Tile shape: prefer the largest omega run per block (T*M) that is mostly
+// used, then enough particle slices to put ~8 waves on each of the 1024 SIMDs.
 Plan make_plan(const sr_grid &g, const sr_opts &o, int64_t n_particles, uint32_t n_snaps) {
   Plan pl;
   const int64_t n_om = g.n_omega;
   const int64_t n_dir = (int64_t)g.n_theta * g.n_phi;
+  pl.sweep = SWEEP_DIRECT;
+  if (o.precision == SR_PREC_SPLIT && g.omega_step != 0.0 && o.sweep == SR_SWEEP_RECURRENCE)
+    pl.sweep = SWEEP_REC;
   int best_w = 1, best_m = 1;
   double best_score = -1;
-  const int m_opts_split[2] = {1, 2};
+  const int m_opts[4] = {1, 2, 4, 8};
   for (int w = 1; w <= 4; w *= 2) {
-    for (int mi = 0; mi < 2; mi++) {
-      const int m = m_opts_split[mi];
-      if (o.precision == SR_PREC_F64 && m > 1) continue;
+    for (int mi = 0; mi < 4; mi++) {
+      const int m = m_opts[mi];
+      if (!allowed_m(g.mode, o.precision, pl.sweep, m)) continue;
       const int64_t run = (int64_t)kWave * w * m;
       const int64_t chunks = (n_om + run - 1) / run;
       const double eff = (double)n_om / (double)(chunks * run);
       if (eff < 0.75) continue;
-      const double score = eff + 0.01 * w + 0.005 * m;
+      // rec: larger M amortises the per-step base sin/cos; M=4 keeps the
+      // Kahan accumulators in registers
+      const double mpref = (pl.sweep == SWEEP_REC) ? (m == 4 ? 0.03 : 0.0) : (m == 2 ? 0.02 : 0.0);
+      const double score = eff + 0.01 * w + mpref;
       if (score > best_score) { best_score = score; best_w = w; best_m = m; }
     }
   }
-  if (best_score < 0) { best_w = 1; best_m = 1; }
+  if (best_score < 0) {
+    best_w = 1;
+    best_m = (pl.sweep == SWEEP_REC) ? 2 : 1;
+  }
   pl.waves = o.waves_per_block > 0 ? o.waves_per_block : best_w;
   pl.M = o.omega_per_lane > 0 ? o.omega_per_lane : best_m;
-  if (o.precision == SR_PREC_F64) pl.M = 1;
+  if (!allowed_m(g.mode, o.precision, pl.sweep, pl.M)) pl.M = (pl.sweep == SWEEP_REC) ? 2 : 1;
   const int64_t run = (int64_t)kWave * pl.waves * pl.M;
   const int64_t chunks = (n_om + run - 1) / run;
   const int64_t base_waves = chunks * n_dir * pl.waves;
   int s = 1;
   if (o.slices > 0) {
     s = o.slices;
-  } else if (n_snaps == 1) {
+  } else {
     const int64_t target = 8 * 1024;
     while (s < kMaxSlices && base_waves * s < target && (int64_t)(s * 2) <= n_particles) s *= 2;
   }
-  if (n_snaps != 1) s = 1;
   if (s > n_particles) s = (int)std::max<int64_t>(1, n_particles);
   pl.slices = s;
   pl.blocks = chunks * n_dir * s;
+  (void)n_snaps;
   return pl;
 }
 
-template <int MODE, int PREC, int M>
+template <int MODE, int PREC, int SWEEP, int M>
 hipError_t launch_t(const KParams &kp, int64_t grid, int threads, hipStream_t st) {
-  hipLaunchKernelGGL((k_spectrum<MODE, PREC, M>), dim3((unsigned)grid), dim3(threads), 0, st, kp);
+  const size_t lds = (PREC == SR_PREC_SPLIT) ? kahan_lds_bytes<MODE, M>(threads) : 0;
+  hipLaunchKernelGGL((k_spectrum<MODE, PREC, SWEEP, M>), dim3((unsigned)grid), dim3(threads), lds, st,
+                     kp);
   return hipGetLastError();
 }
 
-hipError_t launch_tiles(int mode, int prec, int M, const KParams &kp, int64_t grid, int threads,
-                        hipStream_t st) {
-  if (mode == SR_MODE_FAR) {
-    if (prec == SR_PREC_F64) return launch_t<SR_MODE_FAR, SR_PREC_F64, 1>(kp, grid, threads, st);
-    if (M == 2) return launch_t<SR_MODE_FAR, SR_PREC_SPLIT, 2>(kp, grid, threads, st);
-    return launch_t<SR_MODE_FAR, SR_PREC_SPLIT, 1>(kp, grid, threads, st);
+template <int MODE>
+hipError_t launch_mode(const Plan &pl, int prec, const KParams &kp, int64_t grid, int threads,
+                       hipStream_t st) {
+  if (prec == SR_PREC_F64) return launch_t<MODE, SR_PREC_F64, SWEEP_DIRECT, 1>(kp, grid, threads, st);
+  if (pl.sweep == SWEEP_REC) {
+    if (MODE == SR_MODE_NEAR || pl.M == 2)
+      return launch_t<MODE, SR_PREC_SPLIT, SWEEP_REC, 2>(kp, grid, threads, st);
+    return launch_t<MODE, SR_PREC_SPLIT, SWEEP_REC, (MODE == SR_MODE_FAR ? 4 : 2)>(kp, grid, threads, st);
   }
-  if (prec == SR_PREC_F64) return launch_t<SR_MODE_NEAR, SR_PREC_F64, 1>(kp, grid, threads, st);
-  if (M == 2) return launch_t<SR_MODE_NEAR, SR_PREC_SPLIT, 2>(kp, grid, threads, st);
-  return launch_t<SR_MODE_NEAR, SR_PREC_SPLIT, 1>(kp, grid, threads, st);
+  if (MODE == SR_MODE_FAR && pl.M == 4)
+    return launch_t<MODE, SR_PREC_SPLIT, SWEEP_DIRECT, (MODE == SR_MODE_FAR ? 4 : 2)>(kp, grid, threads, st);
+  if (pl.M == 2) return launch_t<MODE, SR_PREC_SPLIT, SWEEP_DIRECT, 2>(kp, grid, threads, st);
+  return launch_t<MODE, SR_PREC_SPLIT, SWEEP_DIRECT, 1>(kp, grid, threads, st);
 }
 
 int check_grid(const sr_grid *g) {
@@ -574,6 +692,7 @@ int check_grid(const sr_grid *g) {
   if (g->mode == SR_MODE_FAR && (!g->sin_theta || !g->cos_theta))
     return fail(SR_ERR_INVALID, "far grid needs sin_theta and cos_theta");
   if (g->mode == SR_MODE_NEAR && !g->radius) return fail(SR_ERR_INVALID, "near grid needs radius");
+  if (!isfinite(g->omega_step)) return fail(SR_ERR_INVALID, "omega_step must be finite");
   const double npts = (double)g->n_omega * g->n_theta * g->n_phi;
   if (npts > 4.0e9) return fail(SR_ERR_INVALID, "grid has more than 4e9 points");
   return SR_OK;
@@ -585,14 +704,19 @@ int check_opts(const sr_grid *g, const sr_opts *o) {
     return fail(SR_ERR_INVALID, "comp %d is not available in mode %d", o->comp, g->mode);
   if (o->precision != SR_PREC_F64 && o->precision != SR_PREC_SPLIT)
     return fail(SR_ERR_INVALID, "precision %d is not F64 (0) or SPLIT (1)", o->precision);
-  if (o->omega_per_lane < 0 || o->omega_per_lane > 2)
-    return fail(SR_ERR_INVALID, "omega_per_lane must be 0, 1 or 2");
+  if (!(o->omega_per_lane == 0 || o->omega_per_lane == 1 || o->omega_per_lane == 2 ||
+        o->omega_per_lane == 4 || o->omega_per_lane == 8))
+    return fail(SR_ERR_INVALID, "omega_per_lane must be 0, 1, 2, 4 or 8");
   if (!(o->waves_per_block == 0 || o->waves_per_block == 1 || o->waves_per_block == 2 ||
         o->waves_per_block == 4))
     return fail(SR_ERR_INVALID, "waves_per_block must be 0, 1, 2 or 4");
   if (o->slices < 0 || o->slices > kMaxSlices)
     return fail(SR_ERR_INVALID, "slices must be in [0, %d]", kMaxSlices);
   if (o->persistent < 0) return fail(SR_ERR_INVALID, "persistent must be >= 0");
+  if (o->sweep < SR_SWEEP_AUTO || o->sweep > SR_SWEEP_RECURRENCE)
+    return fail(SR_ERR_INVALID, "sweep must be SR_SWEEP_AUTO, _DIRECT or _RECURRENCE");
+  if (o->sweep == SR_SWEEP_RECURRENCE && (o->precision != SR_PREC_SPLIT || g->omega_step == 0.0))
+    return fail(SR_ERR_INVALID, "the recurrence sweep needs SR_PREC_SPLIT and a constant omega_step");
   if (g->mode == SR_MODE_FAR && o->comp == SR_COMP_CARTESIAN_COMPLEX && !o->form_factor)
     return fail(SR_ERR_INVALID, "far cartesian_complex needs form_factor");
   return SR_OK;
@@ -610,6 +734,7 @@ void fill_grid(KParams &kp, const sr_grid &g, const sr_opts &o, int n_out, doubl
   kp.sin_p = g.sin_phi;
   kp.cos_p = g.cos_phi;
   kp.l_screen = g.l_screen;
+  kp.omega_step = g.omega_step;
   kp.form_factor = o.form_factor;
   kp.n_grid = (int64_t)g.n_omega * g.n_theta * g.n_phi;
   kp.n_out = n_out;
@@ -634,6 +759,10 @@ void slice_particles(KParams &kp, int slices, int64_t n_particles, const int64_t
   kp.slice_lo[slices] = n_particles;
 }
 
+size_t workspace_for(const Plan &pl, int n_out, int64_t n_grid, uint32_t n_snaps) {
+  return pl.slices > 1 ? (size_t)pl.slices * n_out * (size_t)n_grid * n_snaps * sizeof(double) : 0;
+}
+
 int run(KParams &kp, const Plan &pl, const sr_grid &g, const sr_opts &o, void *ws, size_t ws_bytes,
         hipStream_t st) {
   const int threads = kWave * pl.waves;
@@ -641,23 +770,28 @@ int run(KParams &kp, const Plan &pl, const sr_grid &g, const sr_opts &o, void *w
   kp.n_chunks = (uint32_t)((g.n_omega + run - 1) / run);
   kp.slices = pl.slices;
   kp.n_tiles = pl.blocks;
-  if (pl.slices > 1) {
-    const size_t need = (size_t)pl.slices * kp.n_out * kp.n_grid * sizeof(double);
+  const size_t need = workspace_for(pl, kp.n_out, kp.n_grid, kp.n_snaps);
+  hipError_t e;
+  if (need) {
     if (!ws || ws_bytes < need)
       return fail(SR_ERR_WORKSPACE, "workspace of %zu bytes needed, %zu given", need, ws_bytes);
     kp.scratch = (double *)ws;
+    const int64_t n = (int64_t)(need / sizeof(double));
+    hipLaunchKernelGGL(k_fill_zero, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, kp.scratch, n);
+    e = hipGetLastError();
+    if (e != hipSuccess) return fail(SR_ERR_HIP, "workspace clear: %s", hipGetErrorString(e));
   } else {
     kp.scratch = nullptr;
   }
   int64_t grid = pl.blocks;
   if (o.persistent > 0) grid = std::min<int64_t>(grid, o.persistent);
   if (grid <= 0) return SR_OK;
-  hipError_t e = launch_tiles(g.mode, o.precision, pl.M, kp, grid, threads, st);
+  e = (g.mode == SR_MODE_FAR) ? launch_mode<SR_MODE_FAR>(pl, o.precision, kp, grid, threads, st)
+                              : launch_mode<SR_MODE_NEAR>(pl, o.precision, kp, grid, threads, st);
   if (e != hipSuccess) return fail(SR_ERR_HIP, "tile kernel launch: %s", hipGetErrorString(e));
-  if (pl.slices > 1) {
-    const int64_t total = (int64_t)kp.n_out * kp.n_grid;
-    const int tb = 256;
-    hipLaunchKernelGGL(k_reduce_slices, dim3((unsigned)((total + tb - 1) / tb)), dim3(tb), 0, st, kp);
+  if (need) {
+    const int64_t total = (int64_t)kp.n_out * kp.n_grid * kp.n_snaps;
+    hipLaunchKernelGGL(k_reduce_slices, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, st, kp);
     e = hipGetLastError();
     if (e != hipSuccess) return fail(SR_ERR_HIP, "slice reduction launch: %s", hipGetErrorString(e));
   }
@@ -712,7 +846,7 @@ int single_particle(int32_t mode, int32_t comp, double *const *outs, const doubl
   kp.single_ite = itEnd;
   fill_grid(kp, g, o, n_out, outs);
   Plan pl = make_plan(g, o, 1, nSnaps);
-  pl.slices = 1;  // one particle: no slices
+  pl.slices = 1;  // one particle: no slices, no workspace
   slice_particles(kp, 1, 1, nullptr);
   return run(kp, pl, g, o, nullptr, 0, (hipStream_t)stream);
 }
@@ -726,7 +860,7 @@ extern "C" {
 
 int sr_abi_version(void) { return SR_ABI_VERSION; }
 
-const char *sr_last_error(void) { return g_err; }
+const char *sr_last_error(void) { return sr::last_error(); }
 
 int sr_num_outputs(int32_t mode, int32_t comp) {
   const int n = outputs_for(mode, comp);
@@ -740,9 +874,8 @@ int sr_workspace_bytes(const sr_tracks *tracks, const sr_grid *grid, const sr_op
   *bytes = 0;
   if ((rc = check_tracks(tracks)) || (rc = check_grid(grid)) || (rc = check_opts(grid, opts))) return rc;
   const Plan pl = make_plan(*grid, *opts, tracks->n_particles, tracks->n_snaps);
-  if (pl.slices > 1)
-    *bytes = (size_t)pl.slices * outputs_for(grid->mode, opts->comp) *
-             ((size_t)grid->n_omega * grid->n_theta * grid->n_phi) * sizeof(double);
+  *bytes = workspace_for(pl, outputs_for(grid->mode, opts->comp),
+                         (int64_t)grid->n_omega * grid->n_theta * grid->n_phi, tracks->n_snaps);
   return SR_OK;
 }
 

@@ -151,9 +151,13 @@ def pack_tracks(tracks: Sequence[Sequence], *, dt: float, it_range=None, nSnaps:
 # device objects
 # ---------------------------------------------------------------------------
 class DeviceBatch:
-    """A HostBatch uploaded to the HIP device (one copy per array)."""
+    """A HostBatch uploaded to the HIP device (one copy per array).
 
-    def __init__(self, hb: HostBatch, device=None):
+    ``tensors`` may supply the coordinate, momentum and weight arrays as
+    existing device tensors (float64, contiguous; x, y, z, ux, uy, uz, weight),
+    which are then used in place -- the autograd path does this."""
+
+    def __init__(self, hb: HostBatch, device=None, tensors: Optional[Sequence[torch.Tensor]] = None):
         dev = _dev(device)
         self.host = hb
         self.device = dev
@@ -165,9 +169,16 @@ class DeviceBatch:
             return torch.from_numpy(np.ascontiguousarray(a, dtype=np.uint32).view(np.int32)).to(dev)
 
         self.offsets = torch.from_numpy(hb.offsets).to(dev)
-        self.x, self.y, self.z = f64(hb.x), f64(hb.y), f64(hb.z)
-        self.ux, self.uy, self.uz = f64(hb.ux), f64(hb.uy), f64(hb.uz)
-        self.weight = f64(hb.weight)
+        if tensors is not None:
+            ts = list(tensors)
+            for t, n in zip(ts, [hb.n_points] * 6 + [hb.n_particles]):
+                if t.dtype != torch.float64 or not t.is_contiguous() or t.numel() != n or t.device != dev:
+                    raise ValueError("track tensors must be contiguous float64 on the batch device")
+            self.x, self.y, self.z, self.ux, self.uy, self.uz, self.weight = ts
+        else:
+            self.x, self.y, self.z = f64(hb.x), f64(hb.y), f64(hb.z)
+            self.ux, self.uy, self.uz = f64(hb.ux), f64(hb.uy), f64(hb.uz)
+            self.weight = f64(hb.weight)
         self.it_start = u32(hb.it_start)
         self.it_end = u32(hb.it_end)
         self.it_snaps = u32(hb.it_snaps)
@@ -217,7 +228,9 @@ class DeviceGrid:
             return torch.from_numpy(np.ascontiguousarray(a, dtype=np.float64)).to(dev)
 
         self.omega_host = omega
-        self.omega = f64(np.double(2 * np.pi) * omega)
+        w = np.double(2 * np.pi) * omega
+        self.omega = f64(w)
+        self.omega_step = uniform_step(w)
         self.sin_phi = f64(np.sin(phi))
         self.cos_phi = f64(np.cos(phi))
         self.sin_theta = self.cos_theta = self.radius = None
@@ -248,12 +261,37 @@ class DeviceGrid:
         else:
             g.radius = self.radius.data_ptr()
         g.l_screen = self.l_screen
+        g.omega_step = self.omega_step
         return g
 
     def form_factor(self, sigma_particle: float) -> torch.Tensor:
-        """exp(-(2 pi omega sigma)^2 / 2), synchrad/calc.py:475-480."""
-        ff = np.exp(np.double(-0.5) * (np.double(2 * np.pi) * self.omega_host * np.double(sigma_particle)) ** 2)
-        return torch.from_numpy(np.ascontiguousarray(ff)).to(self.device)
+        """exp(-(2 pi omega sigma)^2 / 2), synchrad/calc.py:475-480 (made once
+        per sigma and kept, so a graph capture finds it already on the device)."""
+        key = float(sigma_particle)
+        cache = self.__dict__.setdefault("_ff", {})
+        if key not in cache:
+            ff = np.exp(np.double(-0.5) * (np.double(2 * np.pi) * self.omega_host * np.double(key)) ** 2)
+            cache[key] = torch.from_numpy(np.ascontiguousarray(ff)).to(self.device)
+        return cache[key]
+
+
+def uniform_step(w: np.ndarray, rtol: float = 1e-14) -> float:
+    """The constant step of ``w`` if every sample lies within ``rtol*max|w|``
+    of w[0] + i*step (linear grids built by np.r_/linspace do), else 0.  The
+    recurrence sweep uses it; the bound keeps its phase error below
+    rtol*max|omega*tau|."""
+    n = w.size
+    if n < 2:
+        return 0.0
+    step = (w[-1] - w[0]) / (n - 1)
+    if step == 0.0:
+        return 0.0
+    dev = np.max(np.abs(w - (w[0] + step * np.arange(n))))
+    return float(step) if dev <= rtol * np.max(np.abs(w)) else 0.0
+
+
+SWEEPS = {"auto": _abi.SR_SWEEP_AUTO, "direct": _abi.SR_SWEEP_DIRECT,
+          "recurrence": _abi.SR_SWEEP_RECURRENCE}
 
 
 @dataclass
@@ -262,16 +300,28 @@ class Tuning:
     waves_per_block: int = 0
     slices: int = 0
     persistent: int = 0
+    sweep: str = "auto"
 
 
 def spectrum(batch: DeviceBatch, grid: DeviceGrid, comp: str = "total", precision="double",
              outs: Optional[List[torch.Tensor]] = None, sigma_particle: float = 0.0,
-             tuning: Optional[Tuning] = None, stream=None) -> List[torch.Tensor]:
+             tuning=None, stream=None, workspace: Optional[torch.Tensor] = None) -> List[torch.Tensor]:
     """Add one batch's spectrum to ``outs`` (allocated zero if None) and return
     them.  Each output is float64 (nSnaps, nPhi, nTheta|nR, nOmega) on the
     device -- the reference's device layout (kernel_farfield.cl:102).
-    Stream-ordered on the current (or given) torch stream; no synchronisation."""
+    Stream-ordered on the current (or given) torch stream; no synchronisation.
+
+    ``tuning``: None (the library's own plan), a :class:`Tuning`, or "auto"
+    (the autotuner's plan for this batch and grid, timed once and cached).
+    ``workspace``: a preallocated uint8 device tensor to use instead of
+    allocating one (needed under graph capture)."""
     L = _abi.load()
+    if isinstance(tuning, str):
+        if tuning != "auto":
+            raise ValueError("tuning must be None, a Tuning or 'auto'")
+        from . import autotune
+        prec_name = precision if isinstance(precision, str) else ("double" if precision == 0 else "float")
+        tuning, _ = autotune.tune(batch, grid, comp=comp, precision=prec_name)
     if comp not in _abi.COMPS:
         raise ValueError(f"unknown comp {comp!r}")
     prec = PRECISIONS[precision] if isinstance(precision, str) else int(precision)
@@ -294,6 +344,7 @@ def spectrum(batch: DeviceBatch, grid: DeviceGrid, comp: str = "total", precisio
     opts.waves_per_block = tun.waves_per_block
     opts.slices = tun.slices
     opts.persistent = tun.persistent
+    opts.sweep = SWEEPS[tun.sweep]
     ff = None
     if comp == "cartesian_complex" and grid.mode == "far":
         ff = grid.form_factor(sigma_particle)
@@ -305,7 +356,12 @@ def spectrum(batch: DeviceBatch, grid: DeviceGrid, comp: str = "total", precisio
                                     ctypes.byref(nbytes)))
     ws = None
     if nbytes.value:
-        ws = torch.empty(nbytes.value, dtype=torch.uint8, device=grid.device)
+        if workspace is not None:
+            if workspace.numel() < nbytes.value or workspace.dtype != torch.uint8:
+                raise ValueError(f"workspace needs {nbytes.value} uint8 elements")
+            ws = workspace
+        else:
+            ws = torch.empty(nbytes.value, dtype=torch.uint8, device=grid.device)
     out_ptrs = (ctypes.c_void_p * n_out)(*[o.data_ptr() for o in outs])
     st = stream if stream is not None else torch.cuda.current_stream(grid.device)
     _abi.check(L.sr_spectrum_batch(ctypes.byref(tr), ctypes.byref(gr), ctypes.byref(opts), out_ptrs,
@@ -318,3 +374,115 @@ def spectrum(batch: DeviceBatch, grid: DeviceGrid, comp: str = "total", precisio
             if t is not None:
                 t.record_stream(st)
     return outs
+
+
+def workspace_bytes(batch: DeviceBatch, grid: DeviceGrid, comp: str = "total", precision="double",
+                    tuning: Optional[Tuning] = None) -> int:
+    """Bytes of workspace :func:`spectrum` needs for this request."""
+    L = _abi.load()
+    tun = tuning or Tuning()
+    opts = _abi.SrOpts()
+    opts.comp = _abi.COMPS[comp]
+    opts.precision = PRECISIONS[precision] if isinstance(precision, str) else int(precision)
+    opts.omega_per_lane = tun.omega_per_lane
+    opts.waves_per_block = tun.waves_per_block
+    opts.slices = tun.slices
+    opts.persistent = tun.persistent
+    opts.sweep = SWEEPS[tun.sweep]
+    opts.form_factor = 1 if (comp == "cartesian_complex" and grid.mode == "far") else None
+    tr, gr = batch.abi(), grid.abi()
+    nbytes = ctypes.c_size_t(0)
+    _abi.check(L.sr_workspace_bytes(ctypes.byref(tr), ctypes.byref(gr), ctypes.byref(opts),
+                                    ctypes.byref(nbytes)))
+    return int(nbytes.value)
+
+
+class SpectrumGraph:
+    """One spectrum captured as a HIP graph and replayed.
+
+    Outputs and workspace are allocated once; each replay clears the outputs
+    with a fill kernel (no memset node) and launches the library's kernels
+    exactly as :func:`spectrum` does, so repeated spectra of the same batch
+    (or of a batch whose device arrays were refilled in place) cost one graph
+    launch."""
+
+    def __init__(self, batch: DeviceBatch, grid: DeviceGrid, comp: str = "total", precision="float",
+                 tuning: Optional[Tuning] = None, sigma_particle: float = 0.0):
+        self.batch, self.grid, self.comp, self.precision = batch, grid, comp, precision
+        if isinstance(tuning, str):
+            from . import autotune
+            tuning, _ = autotune.tune(batch, grid, comp=comp, precision=precision)
+        self.tuning = tuning or Tuning()
+        L = _abi.load()
+        n_out = L.sr_num_outputs(_abi.MODES[grid.mode], _abi.COMPS[comp])
+        shape = (batch.host.n_snaps, grid.n_phi, grid.n1, grid.n_omega)
+        self.outs = [torch.zeros(shape, dtype=torch.float64, device=grid.device) for _ in range(n_out)]
+        nb = workspace_bytes(batch, grid, comp, precision, self.tuning)
+        self.workspace = torch.empty(max(nb, 1), dtype=torch.uint8, device=grid.device)
+        self.ff = grid.form_factor(sigma_particle) if (comp == "cartesian_complex" and grid.mode == "far") else None
+        self.sigma = sigma_particle
+        # warm up on a side stream (loads the code objects), then capture
+        s = torch.cuda.Stream(device=grid.device)
+        s.wait_stream(torch.cuda.current_stream(grid.device))
+        with torch.cuda.stream(s):
+            self._body()
+        torch.cuda.current_stream(grid.device).wait_stream(s)
+        torch.cuda.synchronize(grid.device)
+        self.graph = torch.cuda.CUDAGraph()
+        with torch.cuda.graph(self.graph):
+            self._body()
+
+    def _body(self):
+        for o in self.outs:
+            o.zero_()
+        spectrum(self.batch, self.grid, comp=self.comp, precision=self.precision, outs=self.outs,
+                 sigma_particle=self.sigma, tuning=self.tuning, workspace=self.workspace)
+
+    def replay(self) -> List[torch.Tensor]:
+        self.graph.replay()
+        return self.outs
+
+
+def far_adjoint(batch: DeviceBatch, grid: DeviceGrid, dS: List[torch.Tensor], comp: str = "total",
+                precision="double", tuning: Optional[Tuning] = None, grads=None, stream=None):
+    """dL/d(tracks) of the far-field spectrum given ``dS`` = dL/dS (device
+    float64, one array for 'total', three for 'cartesian', each shaped like the
+    spectrum: (1, nPhi, nTheta, nOmega)).  Returns ``(gx, gy, gz, gux, guy, guz,
+    gw)`` device tensors (added to ``grads`` if given)."""
+    L = _abi.load()
+    if grid.mode != "far":
+        raise ValueError("the adjoint covers the far field")
+    n_in = {"total": 1, "cartesian": 3}.get(comp)
+    if n_in is None:
+        raise ValueError("the adjoint covers comp 'total' and 'cartesian'")
+    if len(dS) != n_in:
+        raise ValueError(f"{comp} needs {n_in} dS arrays")
+    shape = (1, grid.n_phi, grid.n1, grid.n_omega)
+    dS = [d.to(dtype=torch.float64).contiguous() for d in dS]
+    for d in dS:
+        if tuple(d.shape) != shape:
+            raise ValueError(f"dS arrays must have shape {shape}")
+    dev = grid.device
+    npts, npart = batch.host.n_points, batch.host.n_particles
+    if grads is None:
+        grads = [torch.zeros(npts, dtype=torch.float64, device=dev) for _ in range(6)] + \
+                [torch.zeros(npart, dtype=torch.float64, device=dev)]
+    tun = tuning or Tuning()
+    opts = _abi.SrOpts()
+    opts.comp = _abi.COMPS[comp]
+    opts.precision = PRECISIONS[precision] if isinstance(precision, str) else int(precision)
+    opts.waves_per_block = tun.waves_per_block
+    opts.slices = tun.slices
+    tr, gr = batch.abi(), grid.abi()
+    nbytes = ctypes.c_size_t(0)
+    _abi.check(L.sr_far_adjoint_workspace_bytes(ctypes.byref(tr), ctypes.byref(gr), ctypes.byref(opts),
+                                                ctypes.byref(nbytes)))
+    ws = torch.empty(max(int(nbytes.value), 1), dtype=torch.uint8, device=dev)
+    dptr = (ctypes.c_void_p * n_in)(*[d.data_ptr() for d in dS])
+    st = stream if stream is not None else torch.cuda.current_stream(dev)
+    _abi.check(L.sr_far_adjoint_batch(ctypes.byref(tr), ctypes.byref(gr), ctypes.byref(opts), dptr,
+                                      *[g.data_ptr() for g in grads], ctypes.c_void_p(ws.data_ptr()),
+                                      nbytes.value, ctypes.c_void_p(st.cuda_stream)))
+    if st != torch.cuda.current_stream(dev):
+        ws.record_stream(st)
+    return tuple(grads)

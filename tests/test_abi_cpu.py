@@ -98,7 +98,10 @@ def _err():
     (dict(opts=dict(comp=9)), "comp"),
     (dict(opts=dict(precision=5)), "precision"),
     (dict(opts=dict(waves_per_block=3)), "waves_per_block"),
-    (dict(opts=dict(omega_per_lane=4)), "omega_per_lane"),
+    (dict(opts=dict(omega_per_lane=3)), "omega_per_lane"),
+    (dict(opts=dict(sweep=5)), "sweep"),
+    (dict(opts=dict(sweep=2)), "recurrence"),
+    (dict(grid=dict(omega_step=float("inf"))), "omega_step"),
     (dict(opts=dict(slices=99)), "slices"),
     (dict(opts=dict(comp=_abi.SR_COMP_CARTESIAN_COMPLEX)), "form_factor"),
     (dict(tracks=dict(n_snaps=0)), "n_snaps"),
@@ -136,9 +139,10 @@ def test_workspace_plan():
     g = _grid(n_omega=128, n_theta=32, n_phi=32)
     rc, n = _ws(_tracks(n_particles=256), g, _opts())
     assert rc == 0 and n > 0 and n % (8 * 128 * 32 * 32) == 0
-    # several snapshots never use slices
-    rc, n = _ws(_tracks(n_particles=256, n_snaps=3, snaps_stride=3), g, _opts())
-    assert rc == 0 and n == 0
+    # several snapshots: the slices hold every snapshot
+    rc, n1 = _ws(_tracks(n_particles=256), g, _opts())
+    rc, n3 = _ws(_tracks(n_particles=256, n_snaps=3, snaps_stride=3), g, _opts())
+    assert rc == 0 and n3 == 3 * n1
     # explicit slices
     rc, n = _ws(_tracks(n_particles=256), g, _opts(slices=3, comp=_abi.SR_COMP_CARTESIAN))
     assert rc == 0 and n == 3 * 3 * 128 * 32 * 32 * 8

@@ -71,13 +71,22 @@ def test_far_total_undulator_full_config(undulator, precision):
         assert max_rel(got["total"], ref["total"]) < 1e-8
 
 
-@pytest.mark.parametrize("precision", ["double", "float"])
-def test_far_total_betatron_subset(precision):
+@pytest.fixture(scope="module")
+def betatron4():
     wl = workloads.config2(n_particles=4, grid=(64, 16, 16))
     tr = wl.track_list()
     ref = oracle.spectrum("far", "total", tr, wl.axes, wl.batch.dt)
-    got = gpu_spectrum(tr, wl.axes, wl.batch.dt, precision=precision)
-    check(got, ref, precision, "betatron-4e")
+    return wl, tr, ref
+
+
+@pytest.mark.parametrize("precision,sweep,M", [("double", "auto", 0), ("float", "direct", 1),
+                                               ("float", "direct", 2), ("float", "recurrence", 2),
+                                               ("float", "direct", 4), ("float", "recurrence", 4)])
+def test_far_total_betatron_subset(betatron4, precision, sweep, M):
+    wl, tr, ref = betatron4
+    got = gpu_spectrum(tr, wl.axes, wl.batch.dt, precision=precision,
+                       tuning=engine.Tuning(sweep=sweep, omega_per_lane=M))
+    check(got, ref, precision, f"betatron-4e {sweep} M={M}")
 
 
 @pytest.mark.parametrize("comp", ["cartesian", "spheric", "cartesian_complex", "spheric_complex"])
@@ -101,13 +110,14 @@ def test_far_components(comp, precision):
 
 
 @pytest.mark.parametrize("comp", ["total", "cartesian", "cartesian_complex"])
-@pytest.mark.parametrize("precision", ["double", "float"])
-def test_near_field(comp, precision):
+@pytest.mark.parametrize("precision,sweep", [("double", "auto"), ("float", "direct"),
+                                             ("float", "recurrence")])
+def test_near_field(comp, precision, sweep):
     wl = workloads.config4(scale=0.25, n_particles=3)
     tr = wl.track_list()
     ref = oracle.spectrum("near", comp, tr, wl.axes, wl.batch.dt, L_screen=wl.L_screen)
     got = gpu_spectrum(tr, wl.axes, wl.batch.dt, mode="near", comp=comp, precision=precision,
-                       L_screen=wl.L_screen)
+                       L_screen=wl.L_screen, tuning=engine.Tuning(sweep=sweep))
     if comp == "cartesian_complex":
         scale = max(np.linalg.norm(ref[k]) for k in ref)
         for k in ref:
@@ -132,15 +142,18 @@ def _ragged_tracks(n=7, seed=3):
     return tr
 
 
-@pytest.mark.parametrize("precision", ["double", "float"])
+@pytest.mark.parametrize("precision,sweep", [("double", "auto"), ("float", "direct"),
+                                             ("float", "recurrence")])
 @pytest.mark.parametrize("it_range,nSnaps", [(None, 1), (None, 3), ((0, 700), 4), ((100, 650), 3)])
-def test_ragged_batch_snapshots(precision, it_range, nSnaps):
+@pytest.mark.parametrize("slices", [0, 3])
+def test_ragged_batch_snapshots(precision, sweep, it_range, nSnaps, slices):
     tr = _ragged_tracks()
     axes = oracle.make_axes([(100.0, 4e4), (0.0, 0.03), (0.0, 2 * np.pi), (40, 6, 5)], "far")
     dt = 0.01 / 24
     ref = oracle.spectrum("far", "total", tr, axes, dt, it_range=it_range, nSnaps=nSnaps)
-    got = gpu_spectrum(tr, axes, dt, precision=precision, it_range=it_range, nSnaps=nSnaps)
-    check(got, ref, precision, f"ragged it_range={it_range} nSnaps={nSnaps}")
+    got = gpu_spectrum(tr, axes, dt, precision=precision, it_range=it_range, nSnaps=nSnaps,
+                       tuning=engine.Tuning(sweep=sweep, slices=slices))
+    check(got, ref, precision, f"ragged it_range={it_range} nSnaps={nSnaps} slices={slices}")
 
 
 @pytest.mark.parametrize("precision", ["double", "float"])
@@ -160,6 +173,8 @@ def test_uneven_csr_slices_agree(precision):
 @pytest.mark.parametrize("tun", [engine.Tuning(waves_per_block=1), engine.Tuning(waves_per_block=2),
                                  engine.Tuning(omega_per_lane=1, waves_per_block=4),
                                  engine.Tuning(omega_per_lane=2, waves_per_block=2),
+                                 engine.Tuning(omega_per_lane=4, waves_per_block=1),
+                                 engine.Tuning(sweep="recurrence", omega_per_lane=4, waves_per_block=2),
                                  engine.Tuning(persistent=37)])
 def test_tile_shapes_agree(tun):
     wl = workloads.config2(n_particles=2, n_steps=1500, grid=(200, 6, 4))
@@ -236,3 +251,91 @@ def test_empty_and_degenerate_batches():
     z = np.zeros(1)
     got = gpu_spectrum([[z, z, z, z, z, z + 10.0, 1.0, 0]], axes, dt)   # one sample: no interval
     assert np.all(got["total"] == 0)
+
+
+@pytest.mark.parametrize("comp,grid_shape", [("total", (256, 16, 8)), ("cartesian", (32, 4, 4)),
+                                             ("cartesian_complex", (32, 4, 4))])
+def test_graph_replays_match_eager(comp, grid_shape):
+    """A captured spectrum replayed several times, with eager work and
+    synchronisations in between, equals the eager result every time (the
+    small grid makes the plan use particle slices: workspace clear + reduce)."""
+    wl = workloads.config2(n_particles=6, n_steps=1200, grid=grid_shape)
+    batch = engine.DeviceBatch(wl.batch)
+    ax = wl.axes
+    grid = engine.DeviceGrid("far", ax["omega"], ax["theta"], ax["phi"])
+    eager = [o.clone() for o in engine.spectrum(batch, grid, comp=comp, precision="float",
+                                                 sigma_particle=1e-11)]
+    g = engine.SpectrumGraph(batch, grid, comp=comp, precision="float", sigma_particle=1e-11)
+    for _ in range(4):
+        outs = g.replay()
+        torch.cuda.synchronize()
+        for a, b in zip(outs, eager):
+            assert torch.equal(a, b)
+        engine.spectrum(batch, grid, comp=comp, precision="double")
+        torch.cuda.synchronize()
+
+
+def test_autotuned_plan_matches_oracle():
+    from synchrad_amd import autotune
+    wl = workloads.config2(n_particles=3, n_steps=2000, grid=(128, 8, 8))
+    tr = wl.track_list()
+    ref = oracle.spectrum("far", "total", tr, wl.axes, wl.batch.dt)
+    batch = engine.DeviceBatch(wl.batch)
+    ax = wl.axes
+    grid = engine.DeviceGrid("far", ax["omega"], ax["theta"], ax["phi"])
+    plan, rows = autotune.tune(batch, grid, comp="total", precision="float", target_evals=1e8)
+    assert len(rows) >= 9 and all(r["ms"] > 0 for r in rows)
+    out = engine.spectrum(batch, grid, comp="total", precision="float", tuning=plan)[0]
+    got = out.cpu().numpy().swapaxes(-1, -3)
+    assert rel_l2(got, ref["total"]) < SPLIT_TOL
+    print(f"autotuned plan {plan}", flush=True)
+
+
+def test_reference_test_script_runs_unchanged():
+    """tests/test_undulator_analytic.py of the reference, through the drop-in
+    import path, in double and then (after the reference's own re-init calls)
+    in float; both energies within 3% of the analytic estimate."""
+    from synchrad.calc import SynchRad
+    from synchrad.utils import J_in_um
+    K0, Periods, Np, g0 = 0.1, 50, 24, 100.0
+    dg = 1e-4 * g0
+    gg = g0 / (1.0 + K0 ** 2 / 2) ** 0.5
+    k_res = 2 * gg ** 2
+    dt = 1.0 / 32
+    steps = int((Periods + 2) / dt) + 1
+
+    def uxf(z):
+        val = K0 * np.sin(2 * np.pi * z)
+        val *= (z > 0) * (z < 1.5) * z / 1.5 + (z > 1.5)
+        val *= (z > Periods - 1.5) * (z < Periods) * (Periods - z) / 1.5 + (z < Periods - 1.5)
+        return val
+
+    t = np.r_[-1:Periods + 1:steps * 1j]
+    rng = np.random.default_rng(11)
+    tracks = []
+    for g0p in g0 + dg * rng.standard_normal(Np):
+        ggp = g0p / (1.0 + K0 ** 2 / 2) ** 0.5
+        vb = (1.0 - ggp ** -2) ** 0.5
+        z = vb * t
+        ux = uxf(z - 0.5 * dt)
+        uz = (g0 ** 2 - 1 - ux ** 2) ** 0.5
+        x = ux[0] / g0p * dt / 2 + np.cumsum(ux / g0p) * dt
+        tracks.append([x, np.zeros_like(t), z, ux, np.zeros_like(t), uz, 1.0, 0])
+    calc = SynchRad({"grid": [(0.02 * k_res, 1.1 * k_res), (0, 2.0 / g0), (0.0, 2 * np.pi), (128, 32, 32)]})
+    e_th = Np * k_res * J_in_um * (7 * np.pi / 24) / 137.0 * K0 ** 2 * (1 + K0 ** 2 / 2) * Periods
+    calc.calculate_spectrum(tracks.copy(), timeStep=dt, comp="total", Np_max=Np)
+    e64 = calc.get_energy(lambda0_um=1)
+    ref = oracle.spectrum("far", "total", tracks, {"omega": calc.Args["omega"], "theta": calc.Args["theta"],
+                                                   "phi": calc.Args["phi"]}, dt)["total"]
+    assert rel_l2(calc.Data["radiation"]["total"], ref) < F64_TOL
+    calc.Args["dtype"] = "float"
+    calc.Args["native"] = True
+    calc._init_args(calc.Args)
+    calc._init_data()
+    calc._compile_kernels()
+    calc.calculate_spectrum(tracks.copy(), timeStep=dt, comp="total", Np_max=Np)
+    e32 = calc.get_energy(lambda0_um=1)
+    for e in (e64, e32):
+        print(f"deviation from the analytic estimate: {abs(e - e_th) / e_th:.4f}", flush=True)
+        assert abs(e - e_th) / e_th < 0.03
+    assert abs(e32 - e64) / e64 < 1e-6

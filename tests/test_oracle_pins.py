@@ -387,8 +387,9 @@ MUTANTS = {
                      "re = add3(re, add3(scl3(sinPhase, c1), scl3(cosPhase, c2)));"),
     "spheric_theta_sign": ("const v3 thVec = mk(cosTheta[iTheta] * cosPhi[iPhi], cosTheta[iTheta] * sinPhi[iPhi],\n                        -sinTheta[iTheta]);",
                            "const v3 thVec = mk(cosTheta[iTheta] * cosPhi[iPhi], cosTheta[iTheta] * sinPhi[iPhi],\n                        sinTheta[iTheta]);"),
-    "snapshot_off_by_one": ("it_glob + 2 == itSnaps[iSnap]) {\n        record(comp, spectra, (size_t)(gti + nTotal * iSnap), wpdt2, wpdt, re, im);\n        iSnap += 1;\n      }\n    }\n  }\n  return 0;\n}\n\n/*",
-                            "it_glob + 1 == itSnaps[iSnap]) {\n        record(comp, spectra, (size_t)(gti + nTotal * iSnap), wpdt2, wpdt, re, im);\n        iSnap += 1;\n      }\n    }\n  }\n  return 0;\n}\n\n/*"),
+    # first occurrence: the far-field kernel's record test
+    "snapshot_off_by_one": ("if (iSnap < nSnaps && it_glob + 2 == itSnaps[iSnap]) {",
+                            "if (iSnap < nSnaps && it_glob + 1 == itSnaps[iSnap]) {"),
 }
 
 
@@ -397,9 +398,9 @@ MUTANTS = {
 def test_planted_mistakes_are_caught(name, tmp_path):
     src = open(os.path.join(os.path.dirname(oracle.__file__), "oracle.c")).read()
     old, new = MUTANTS[name]
-    assert src.count(old) == 1, f"mutation site for {name} not found exactly once"
+    assert src.count(old) >= 1, f"mutation site for {name} not found"
     mut = tmp_path / "oracle_mut.c"
-    mut.write_text(src.replace(old, new))
+    mut.write_text(src.replace(old, new, 1))
     so = tmp_path / "liboracle_mut.so"
     subprocess.run(["gcc", "-O2", "-fPIC", "-fopenmp", "-ffp-contract=off", "-shared", "-o", str(so),
                     str(mut), "-lm"], check=True)
