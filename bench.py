@@ -1,25 +1,32 @@
 #!/usr/bin/env python3
 """Throughput of the spectral radiation integral on one MI355X.
 
-Default workload: the betatron beam (2000 electrons x 8000 steps, far field
-'total', 512 omega x 64 theta x 64 phi = 3.355e13 point-step evaluations per
-spectrum) in split precision.  One "step" is one full spectrum of that batch,
-with tracks and grid already resident in HBM.  The FP64 path is timed on a
-200-particle share of the same beam (evals/s is intensive in the particle
-count), and the split result of that share is compared with the FP64 result
-(relative L2).
+Default workload (configuration 2 of the benchmark set): the betatron beam,
+2000 electrons x 8000 steps, far field 'total', 512 omega x 64 theta x 64 phi
+= 3.355e13 point-step evaluations per spectrum, split precision.  One "step"
+is one full spectrum of that batch with tracks and grid resident in HBM.  The
+FP64 path is timed on a 200-particle share of the same beam (evals/s does not
+depend on the particle count) and the split result of that share is compared
+with the FP64 one (relative L2).
 
 Prints one JSON line.  ``roofline`` prices the tile kernel at the reference
-integrand's algorithmic cost (49 flop per far-field evaluation, 54 near) against
-MI355X's FP32 vector peak; ``cpu_baseline`` times the CPU FP64 oracle (a
-restatement of the reference kernels, not the reference itself) on a bounded
+integrand's algorithmic cost (49 flop per far-field evaluation, 54 near)
+against MI355X's FP32 vector peak; ``cpu_baseline`` times the CPU FP64 oracle
+(a C restatement of the reference kernels, not the reference) on a bounded
 share of the same workload on this host.
+
+Other configurations (--config 1..5) print their own line: 1 undulator (FP64),
+3 uneven CSR batch (cartesian; also one launch per particle on a share),
+4 near field, 5 forward + adjoint of a spectral loss.
+``--copy-calibration BYTES`` only runs a device-to-device copy of BYTES (for
+calibrating rocprofv3 FETCH_SIZE / WRITE_SIZE counters).
 """
 from __future__ import annotations
 
 import argparse
 import json
 import os
+import re
 import sys
 import time
 
@@ -32,6 +39,7 @@ METRIC = "G spectral-pt×step evals/s: split-FP32 vs FP64 @≤1e-4 relL2; % vect
 FLOP_PER_EVAL = {"far": 49.0, "near": 54.0}
 PEAK_FP32_TFLOPS = 157.3
 PEAK_FP64_TFLOPS = 78.6
+TRAFFIC_FILE = os.path.join(REPO, "profiles", "tile_kernel_betatron_traffic.md")
 
 
 def log(msg):
@@ -44,21 +52,25 @@ def parse():
     ap.add_argument("--steps", type=int, default=2)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--config", type=int, default=2, help="workload number (1-5), default 2")
-    ap.add_argument("--particles", type=int, default=None, help="override particle count")
+    ap.add_argument("--particles", type=int, default=None, help="override the particle count")
     ap.add_argument("--f64-particles", type=int, default=200)
     ap.add_argument("--no-f64", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-autotune", action="store_true")
     ap.add_argument("--cpu-particles", type=int, default=4)
     ap.add_argument("--cpu-grid", type=str, default="128,32,16")
+    ap.add_argument("--per-particle-share", type=int, default=400,
+                    help="config 3: particles timed with one launch each")
+    ap.add_argument("--copy-calibration", type=int, default=0)
     return ap.parse_args()
 
 
 def build_workload(cfg: int, n_particles=None):
     from synchrad_amd import workloads
-    if cfg == 2:
-        return workloads.config2(n_particles=n_particles)
     fn = workloads.CONFIGS[cfg]
-    return fn(n_particles=n_particles) if n_particles else fn()
+    if cfg == 1:
+        return fn()
+    return fn(n_particles=n_particles)
 
 
 def device_objects(wl):
@@ -70,19 +82,18 @@ def device_objects(wl):
     return batch, grid
 
 
-def time_spectra(batch, grid, comp, precision, steps, warmup):
-    """Mean wall ms per spectrum (HIP events around `steps` launches on the
-    launch stream), outputs of the last spectrum."""
+def time_spectra(batch, grid, comp, precision, steps, warmup, tuning=None):
+    """Mean ms per spectrum (HIP events around `steps` launches on the launch
+    stream, outputs cleared inside the timed region), wall ms, outputs."""
     import torch
     from synchrad_amd import engine
     st = torch.cuda.current_stream()
-    outs = None
+    outs = engine.spectrum(batch, grid, comp=comp, precision=precision, tuning=tuning)
     for _ in range(warmup):
-        outs = engine.spectrum(batch, grid, comp=comp, precision=precision)
+        for o in outs:
+            o.zero_()
+        engine.spectrum(batch, grid, comp=comp, precision=precision, outs=outs, tuning=tuning)
     torch.cuda.synchronize()
-    if outs is None:
-        outs = engine.spectrum(batch, grid, comp=comp, precision=precision)
-        torch.cuda.synchronize()
     t0 = torch.cuda.Event(enable_timing=True)
     t1 = torch.cuda.Event(enable_timing=True)
     wall0 = time.perf_counter()
@@ -90,65 +101,89 @@ def time_spectra(batch, grid, comp, precision, steps, warmup):
     for _ in range(steps):
         for o in outs:
             o.zero_()
-        engine.spectrum(batch, grid, comp=comp, precision=precision, outs=outs)
+        engine.spectrum(batch, grid, comp=comp, precision=precision, outs=outs, tuning=tuning)
     t1.record(st)
     torch.cuda.synchronize()
     wall = (time.perf_counter() - wall0) * 1e3 / steps
-    ms = t0.elapsed_time(t1) / steps
-    return ms, wall, outs
+    return t0.elapsed_time(t1) / steps, wall, outs
 
 
-def cpu_baseline(wl_full, n_particles, grid_str):
+def rel_l2_dev(a_list, b_list):
+    import torch
+    num = sum(float(torch.sum((a - b) ** 2)) for a, b in zip(a_list, b_list))
+    den = sum(float(torch.sum(b ** 2)) for b in b_list)
+    return (num / den) ** 0.5 if den > 0 else float("nan")
+
+
+def traffic_from_profile():
+    """HBM bytes per launch of the tile kernel on the default workload, as
+    measured by rocprofv3 FETCH_SIZE/WRITE_SIZE and recorded in profiles/."""
+    try:
+        text = open(TRAFFIC_FILE).read()
+    except OSError:
+        return None
+    m = re.search(r"traffic_bytes_per_launch:\s*([0-9.eE+]+)", text)
+    return float(m.group(1)) if m else None
+
+
+def cpu_baseline(wl_full, cfg, n_particles, grid_str):
     """FP64 oracle on this host: `n_particles` of the workload on a reduced grid."""
     from oracle import oracle
     from synchrad_amd import workloads
     no, nt, nph = (int(v) for v in grid_str.split(","))
-    if wl_full.name == "betatron":
+    if cfg == 2:
         wl = workloads.config2(n_particles=n_particles, grid=(no, nt, nph))
+        sample_grid = f"{no}x{nt}x{nph}"
     else:
         wl = wl_full
+        sample_grid = "x".join(str(v) for v in wl.calc_input["grid"][-1])
     tr = wl.track_list()[:n_particles]
-    hb_int = sum(max(min(t[0].size - 1, t[0].size - 1), 0) for t in tr)
     ax = wl.axes
-    evals = ax["omega"].size * ax["phi"].size * (ax.get("theta", ax.get("radius"))).size * hb_int
+    n1 = (ax["theta"] if wl.mode == "far" else ax["radius"]).size
+    evals = ax["omega"].size * n1 * ax["phi"].size * sum(max(t[0].size - 1, 0) for t in tr)
     threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
     t0 = time.perf_counter()
     oracle.spectrum(wl.mode, wl.comp, tr, ax, wl.batch.dt, L_screen=wl.L_screen, threads=threads)
     sec = time.perf_counter() - t0
-    return {"value": evals / sec / 1e9, "unit": "G evals/s", "cores": threads, "kind": "port",
-            "sample": f"{len(tr)} particles x {tr[0][0].size} steps on a {no}x{nt}x{nph} grid "
+    return {"value": round(evals / sec / 1e9, 4), "unit": "G evals/s", "cores": threads, "kind": "port",
+            "sample": f"{len(tr)} particles x {tr[0][0].size} steps on a {sample_grid} grid "
                       f"({evals:.3g} evals, {sec:.1f} s), FP64 C/OpenMP restatement of the "
                       f"reference kernels"}
 
 
-def main():
-    args = parse()
-    if args.gpus != 1:
-        print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1 (multi-GPU "
-                                   "sharding is designed but not built)"}))
-        sys.exit(2)
-    import torch
-    from synchrad_amd import engine, _abi
-    assert torch.cuda.is_available(), "bench.py needs a HIP device"
-    torch.cuda.set_device(0)
-    _abi.load()
+def plan_for(batch, grid, comp, precision, args):
+    from dataclasses import asdict
+    from synchrad_amd import autotune
+    if args.no_autotune:
+        return None, None
+    t0 = time.perf_counter()
+    plan, rows = autotune.tune(batch, grid, comp=comp, precision=precision)
+    log(f"autotune ({time.perf_counter() - t0:.1f} s): {asdict(plan)}")
+    return plan, asdict(plan)
 
-    log(f"building workload config {args.config}")
-    wl = build_workload(args.config, args.particles)
+
+def bench_forward(args):
+    import torch
+    from synchrad_amd import engine
+    cfg = args.config
+    log(f"building workload config {cfg}")
+    wl = build_workload(cfg, args.particles)
     evals = wl.evals
     log(f"workload {wl.name}: {wl.batch.n_particles} particles, {wl.n_grid} points, {evals:.4g} evals")
     batch, grid = device_objects(wl)
     torch.cuda.synchronize()
-
-    split_prec = "float"
-    log("timing split precision")
-    ms, wall, outs = time_spectra(batch, grid, wl.comp, split_prec, args.steps, args.warmup)
-    value = evals / (ms * 1e-3) / 1e9
     flop = FLOP_PER_EVAL[wl.mode]
-    achieved_tf = evals * flop / (ms * 1e-3) / 1e12
-    log(f"split: {ms:.1f} ms/spectrum ({wall:.1f} ms wall), {value:.1f} G evals/s, "
-        f"{achieved_tf:.1f} TFLOP/s algorithmic")
+    main_prec = "double" if cfg == 1 else "float"
+    plan, plan_d = plan_for(batch, grid, wl.comp, main_prec, args)
 
+    log(f"timing {main_prec}")
+    ms, wall, outs = time_spectra(batch, grid, wl.comp, main_prec, args.steps, args.warmup, plan)
+    value = evals / (ms * 1e-3) / 1e9
+    achieved_tf = evals * flop / (ms * 1e-3) / 1e12
+    peak = PEAK_FP64_TFLOPS if main_prec == "double" else PEAK_FP32_TFLOPS
+    log(f"{main_prec}: {ms:.1f} ms/spectrum ({wall:.1f} ms wall), {value:.1f} G evals/s, "
+        f"{achieved_tf:.1f} TFLOP/s algorithmic")
+    nmax = int(np.diff(wl.batch.offsets).max())
     res = {
         "metric": METRIC,
         "value": round(value, 3),
@@ -160,41 +195,156 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f32 (f64 phase)",
+        "dtype": "f64" if main_prec == "double" else "f32 (f64 phase)",
         "data": "synthetic",
-        "config": {"workload": f"{wl.name}: {wl.batch.n_particles} e- x "
-                               f"{int(np.diff(wl.batch.offsets).max())} steps, {wl.mode} field "
-                               f"'{wl.comp}', grid {'x'.join(str(v) for v in wl.calc_input['grid'][-1])}",
-                   "evals_per_spectrum": evals, "precision": "split-FP32"},
-        "roofline": {"bound": "valu", "achieved": round(achieved_tf, 3), "peak": PEAK_FP32_TFLOPS,
-                     "unit": "TFLOP/s", "frac": round(achieved_tf / PEAK_FP32_TFLOPS, 4),
-                     "traffic": None,
+        "config": {"workload": f"{wl.name}: {wl.batch.n_particles} e- x up to {nmax} steps, {wl.mode} "
+                               f"field '{wl.comp}', grid "
+                               f"{'x'.join(str(v) for v in wl.calc_input['grid'][-1])}",
+                   "evals_per_spectrum": evals,
+                   "precision": "FP64" if main_prec == "double" else "split-FP32",
+                   "plan": plan_d},
+        "roofline": {"bound": "valu", "achieved": round(achieved_tf, 3), "peak": peak,
+                     "unit": "TFLOP/s", "frac": round(achieved_tf / peak, 4),
+                     "traffic": traffic_from_profile() if cfg == 2 else None,
                      "basis": f"{flop:g} algorithmic flop per point-step evaluation x "
                               f"{evals:.4g} evaluations per launch / mean launch time"},
     }
 
-    if not args.no_f64:
+    if main_prec == "float" and not args.no_f64:
         n64 = min(args.f64_particles, wl.batch.n_particles)
-        wl64 = build_workload(args.config, n64) if n64 < wl.batch.n_particles else wl
+        wl64 = build_workload(cfg, n64) if n64 < wl.batch.n_particles else wl
         b64, g64 = (batch, grid) if wl64 is wl else device_objects(wl64)
         log(f"timing FP64 on {n64} particles")
         ms64, _, outs64 = time_spectra(b64, g64, wl.comp, "double", 1, 0)
         ev64 = wl64.evals
         v64 = ev64 / (ms64 * 1e-3) / 1e9
-        _, _, outs_s = time_spectra(b64, g64, wl.comp, "float", 1, 0)
-        num = sum(float(torch.sum((a - b) ** 2)) for a, b in zip(outs_s, outs64))
-        den = sum(float(torch.sum(b ** 2)) for b in outs64)
-        rel = (num / den) ** 0.5 if den > 0 else float("nan")
+        _, _, outs_s = time_spectra(b64, g64, wl.comp, "float", 1, 0, plan)
+        rel = rel_l2_dev(outs_s, outs64)
         log(f"FP64: {ms64:.1f} ms ({v64:.1f} G evals/s); split vs FP64 rel-L2 {rel:.3e}")
         res["f64"] = {"value": round(v64, 3), "unit": "G evals/s", "particles": n64,
                       "frac_fp64_peak": round(v64 * 1e9 * flop / 1e12 / PEAK_FP64_TFLOPS, 4)}
         res["split_over_f64"] = round(value / v64, 3)
         res["rel_l2_split_vs_f64"] = rel
 
+    if cfg == 3 and args.per_particle_share > 0:
+        res["per_particle_launches"] = per_particle_compare(wl, grid, plan, args.per_particle_share)
+
     if not args.no_cpu:
         log("timing CPU oracle sample")
-        res["cpu_baseline"] = cpu_baseline(wl, args.cpu_particles, args.cpu_grid)
+        res["cpu_baseline"] = cpu_baseline(wl, cfg, args.cpu_particles, args.cpu_grid)
         log(f"cpu: {res['cpu_baseline']}")
+    return res
+
+
+def per_particle_compare(wl, grid, plan, n_share):
+    """One launch per particle (the reference's pattern, through the batch entry
+    point with one particle per call) vs one launch for the same share."""
+    import torch
+    from synchrad_amd import engine
+    hb = wl.batch
+    n = min(n_share, hb.n_particles)
+    tr = wl.track_list()[:n]
+    share = engine.pack_tracks(tr, dt=hb.dt)
+    b_all = engine.DeviceBatch(share)
+    singles = [engine.DeviceBatch(engine.pack_tracks([t], dt=hb.dt)) for t in tr]
+    evals = grid.n_grid * share.n_intervals()
+    ms_one, _, outs = time_spectra(b_all, grid, wl.comp, "float", 1, 1, plan)
+    st = torch.cuda.current_stream()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    outs2 = [torch.zeros_like(o) for o in outs]
+    engine.spectrum(singles[0], grid, comp=wl.comp, precision="float", outs=outs2, tuning=plan)
+    for o in outs2:
+        o.zero_()
+    torch.cuda.synchronize()
+    e0.record(st)
+    for b in singles:
+        engine.spectrum(b, grid, comp=wl.comp, precision="float", outs=outs2, tuning=plan)
+    e1.record(st)
+    torch.cuda.synchronize()
+    ms_many = e0.elapsed_time(e1)
+    return {"particles": n, "evals": evals,
+            "one_launch_G_evals_s": round(evals / (ms_one * 1e-3) / 1e9, 3),
+            "per_particle_G_evals_s": round(evals / (ms_many * 1e-3) / 1e9, 3),
+            "speedup": round(ms_many / ms_one, 3),
+            "rel_l2": rel_l2_dev(outs2, outs)}
+
+
+def bench_adjoint(args):
+    """Configuration 5: forward + adjoint of a weighted L2 spectral loss."""
+    import torch
+    from synchrad_amd import autograd, engine, workloads
+    wl = workloads.config5(n_particles=args.particles)
+    tgt = workloads.config5(n_particles=args.particles, target=True)
+    ax = wl.axes
+    grid = engine.DeviceGrid("far", ax["omega"], ax["theta"], ax["phi"])
+    dev = grid.device
+    hb = wl.batch
+    evals = wl.evals
+    T = engine.spectrum(engine.DeviceBatch(tgt.batch), grid, comp="total", precision="double")[0]
+    th = torch.as_tensor(ax["theta"], device=dev)
+    W = (torch.sin(th)[None, None, :, None] * torch.ones_like(T))
+    W = W / W.sum()
+    leaves = [torch.tensor(getattr(hb, c), device=dev, requires_grad=True)
+              for c in ("x", "y", "z", "ux", "uy", "uz", "weight")]
+    res = {"metric": "forward+backward ms, spectral-loss adjoint", "unit": "ms", "n_gpus": 1,
+           "higher_is_better": False, "data": "synthetic",
+           "config": {"workload": f"adjoint-betatron: {hb.n_particles} e- x 4000 steps (fp32 tracks), "
+                                  "far 'total', grid 128x32x32", "evals_forward": evals}}
+    for prec in ("float", "double"):
+        def step():
+            for t in leaves:
+                t.grad = None
+            S = autograd.far_spectrum(*leaves, hb, grid, comp="total", precision=prec)[0]
+            loss = torch.sum(W * (S - T) ** 2)
+            loss.backward()
+            return loss
+        step()
+        torch.cuda.synchronize()
+        st = torch.cuda.current_stream()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record(st)
+        for _ in range(args.steps):
+            step()
+        e1.record(st)
+        torch.cuda.synchronize()
+        ms = e0.elapsed_time(e1) / args.steps
+        res[prec] = {"ms_fwd_bwd": round(ms, 2),
+                     "G_evals_per_s_fwd_equiv": round(evals / (ms * 1e-3) / 1e9, 2)}
+        log(f"adjoint {prec}: {ms:.1f} ms forward+backward")
+    res["value"] = res["float"]["ms_fwd_bwd"]
+    res["ms_per_step"] = res["value"]
+    res["steps"], res["warmup"] = args.steps, 1
+    res["dtype"] = "f32 (f64 phase)"
+    return res
+
+
+def copy_calibration(nbytes):
+    """A plain device copy of `nbytes` bytes, for counter calibration."""
+    import torch
+    a = torch.empty(nbytes // 4, dtype=torch.float32, device="cuda").fill_(1.0)
+    b = torch.empty_like(a)
+    torch.cuda.synchronize()
+    for _ in range(3):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_bytes": nbytes, "launches": 3}), flush=True)
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1 (multi-GPU sharding "
+                                   "is designed but not built)"}))
+        sys.exit(2)
+    import torch
+    from synchrad_amd import _abi
+    assert torch.cuda.is_available(), "bench.py needs a HIP device"
+    torch.cuda.set_device(0)
+    if args.copy_calibration:
+        copy_calibration(args.copy_calibration)
+        return
+    _abi.load()
+    res = bench_adjoint(args) if args.config == 5 else bench_forward(args)
     print(json.dumps(res), flush=True)
 
 

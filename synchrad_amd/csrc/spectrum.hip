@@ -154,7 +154,7 @@ k_spectrum(const KParams p) {
 
   __shared__ __align__(16) double s_tau[kMaxT];
   __shared__ __align__(16) Amp s_amp[NA][kMaxT];
-  __shared__ __align__(16) double s_thr[kRec ? kMaxT : 1];
+  __shared__ __align__(16) double s_thr[kSplit ? kMaxT : 1];
   __shared__ __align__(16) float s_rot[2][kRec ? kMaxT : 1];
   extern __shared__ __align__(16) float s_dyn[];
 
@@ -208,13 +208,14 @@ k_spectrum(const KParams p) {
     for (int k = 0; k < M; k++) om_rev[k] = om[k] * kInv2Pi;
     // rec sweep: the thread's base frequency in revolutions, the block's omega range
     const double om0_rev = om_rev[0];
+    // range of |omega|/2pi over the block (linear grids: the run's end points)
     double w_lo = 0.0, w_hi = 0.0;
     if (kRec) {
       const int64_t first = (int64_t)chunk * T * M;
       const int64_t last = min((int64_t)(chunk + 1) * T * M, (int64_t)p.n_omega) - 1;
-      const double a = p.omega[first], b = p.omega[last];
-      w_lo = fmin(a, b);
-      w_hi = fmax(a, b);
+      const double a = p.omega[first] * kInv2Pi, b = p.omega[last] * kInv2Pi;
+      w_lo = (a * b <= 0.0) ? 0.0 : fmin(fabs(a), fabs(b));
+      w_hi = fmax(fabs(a), fabs(b));
     }
     double *dst[kMaxOut];
 #pragma unroll
@@ -324,7 +325,7 @@ k_spectrum(const KParams p) {
         for (int k = 0; k < M; k++) {
           float cm = cn, sm = sn;
           if (kMasked) {
-            const bool act = om[k] < thr;
+            const bool act = fabs(om_rev[k]) < thr;
             cm = act ? cn : 0.f;
             sm = act ? sn : 0.f;
           }
@@ -400,7 +401,7 @@ k_spectrum(const KParams p) {
           if (MODE == SR_MODE_FAR) {
             // kernel_farfield.cl:65-94
             tau = time - (xs * gx + ys * gy + zs * gz);
-            if (kRec && s > 0) {
+            if (kSplit && s > 0) {
               const double tp = (double)(uint32_t)(pt.its + (uint32_t)(s - 1)) * dt;
               tau_prev = tp - (p.x[gs - 1] * gx + p.y[gs - 1] * gy + p.z[gs - 1] * gz);
             }
@@ -431,7 +432,7 @@ k_spectrum(const KParams p) {
             const double rx = gx - xs, ry = gy - ys, rz = gz - zs;
             const double R = sqrt(rx * rx + ry * ry + rz * rz);
             tau = time + R;
-            if (kRec && s > 0) {
+            if (kSplit && s > 0) {
               const double tp = (double)(uint32_t)(pt.its + (uint32_t)(s - 1)) * dt;
               const double qx = gx - p.x[gs - 1], qy = gy - p.y[gs - 1], qz = gz - p.z[gs - 1];
               tau_prev = tp + sqrt(qx * qx + qy * qy + qz * qz);
@@ -447,12 +448,14 @@ k_spectrum(const KParams p) {
             s_amp[5][tid] = (Amp)(r2 * nz);
           }
           s_tau[tid] = tau;
-          if (kRec) {
-            // |omega tau_s - omega tau_{s-1}| < pi  <=>  omega < pi/|tau_s - tau_{s-1}|
+          if (kSplit) {
+            // |omega tau_s - omega tau_{s-1}| < pi  <=>  |omega|/2pi < (1/2)/|tau_s - tau_{s-1}|
             // (phase_{-1} = 0 at the first step); 1/0 gives +inf: always active
-            const double thr = kPi / fabs(tau - tau_prev);
+            const double thr = 0.5 / fabs(tau - tau_prev);
             s_thr[tid] = thr;
             step_on = thr > w_hi;
+          }
+          if (kRec) {
             float sr, cr;
             sincos_rev(rot_rev * tau, &sr, &cr);
             s_rot[0][tid] = cr;
@@ -478,20 +481,26 @@ k_spectrum(const KParams p) {
 #pragma unroll 2
             for (; j < jend; j++) {
               const double tau = s_tau[j];
+              const double thr = kSplit ? s_thr[j] : 0.0;
               Amp a[NA];
 #pragma unroll
               for (int c = 0; c < NA; c++) a[c] = s_amp[c][j];
 #pragma unroll
               for (int k = 0; k < M; k++) {
-                // FP64: phase in radians and the reference's exact |dphase| < pi
-                // test.  Split: phase in revolutions (omega/2pi precomputed), so
-                // the test reads |dphase| < 1/2 and the fraction feeds the FP32
-                // units directly; decisions can differ from the radian test
-                // only within a few ulp of the boundary.
+                // FP64: phase in radians and the reference's exact test
+                // |phase_s - phase_{s-1}| < pi.  Split: phase in revolutions
+                // (omega/2pi precomputed) and the same inequality in its
+                // threshold form |omega|/2pi < thr_s, staged per step; the two
+                // can disagree only within rounding of the boundary.
                 const double ph = (kSplit ? om_rev[k] : om[k]) * tau;
-                const double d = ph - prev[k];
-                prev[k] = ph;
-                const bool act = valid[k] && (fabs(d) < (kSplit ? 0.5 : kPi));
+                bool act;
+                if (kSplit) {
+                  act = valid[k] && (fabs(om_rev[k]) < thr);
+                } else {
+                  const double d = ph - prev[k];
+                  prev[k] = ph;
+                  act = valid[k] && (fabs(d) < kPi);
+                }
                 if (act) {
                   if (kSplit) {
                     float sn, cn;
