@@ -84,6 +84,9 @@ using namespace sr;
 
 enum { SWEEP_DIRECT = 0, SWEEP_REC = 1 };
 
+template <int V>
+using IC = std::integral_constant<int, V>;
+
 struct KParams {
   // tracks (CSR); offsets == nullptr selects the single-particle form
   const int64_t *offsets;
@@ -135,9 +138,19 @@ constexpr size_t kahan_lds_bytes(int T) {
 #ifndef SR_REC_GROUP
 #define SR_REC_GROUP 4
 #endif
+#ifndef SR_DIRECT_WAVES
+#define SR_DIRECT_WAVES 3
+#endif
+// Split direct sweep: the steps of a tile are put in an order in which every
+// 64-omega run of the block finds the steps that keep any of its omegas first,
+// so a wave sweeps only those (a stable counting sort per tile).
+#ifndef SR_COMPACT
+#define SR_COMPACT 1
+#endif
+
 template <int PREC, int SWEEP, int M>
 struct WavesPerEU {
-  static constexpr int value = (PREC == SR_PREC_F64) ? 3 : (SWEEP == SWEEP_REC ? SR_REC_WAVES : (M >= 4 ? 2 : 3));
+  static constexpr int value = (PREC == SR_PREC_F64) ? 3 : (SWEEP == SWEEP_REC ? SR_REC_WAVES : (M >= 4 ? 2 : SR_DIRECT_WAVES));
 };
 
 template <int MODE, int PREC, int SWEEP, int M>
@@ -151,15 +164,28 @@ k_spectrum(const KParams p) {
   constexpr bool kRec = (SWEEP == SWEEP_REC);
   constexpr int NA = (MODE == SR_MODE_FAR) ? 3 : 6;   // amplitude words per step
   constexpr int NC = 2 * NA;                           // real accumulators per omega
+  constexpr bool kCompact = kSplit && (SR_COMPACT != 0);
+  constexpr int kMaxW = kMaxT / kWave;                 // waves per block
+  constexpr int kMaxRuns = kMaxW * 4;                  // 64-omega runs per block (M <= 4)
 
   __shared__ __align__(16) double s_tau[kMaxT];
   __shared__ __align__(16) Amp s_amp[NA][kMaxT];
   __shared__ __align__(16) double s_thr[kSplit ? kMaxT : 1];
   __shared__ __align__(16) float s_rot[2][kRec ? kMaxT : 1];
   extern __shared__ __align__(16) float s_dyn[];
+  // compaction: smallest |omega|/2pi of each run, its rank, the per-wave
+  // bucket histogram, its scan, and the step count of each rank
+  __shared__ double s_wmin[kCompact ? kMaxRuns : 1];
+  __shared__ int s_rank[kCompact ? kMaxRuns : 1];
+  __shared__ int s_hist[kCompact ? kMaxRuns * kMaxW : 1];
+  __shared__ int s_off[kCompact ? kMaxRuns * kMaxW : 1];
+  __shared__ int s_cntb[kCompact ? kMaxRuns + 1 : 1];
 
   const int T = blockDim.x;
   const int tid = threadIdx.x;
+  const int W = T / kWave;
+  const int wv = tid / kWave;
+  const int lane = tid % kWave;
   float *const s_ks = s_dyn + tid;                       // S[c*M+k] at s_ks[(c*M+k)*T]
   float *const s_kc = s_dyn + (size_t)NC * M * T + tid;  // C likewise
   const double dt = p.dt;
@@ -210,12 +236,38 @@ k_spectrum(const KParams p) {
     const double om0_rev = om_rev[0];
     // range of |omega|/2pi over the block (linear grids: the run's end points)
     double w_lo = 0.0, w_hi = 0.0;
+    bool rec_sortable = false;
     if (kRec) {
       const int64_t first = (int64_t)chunk * T * M;
       const int64_t last = min((int64_t)(chunk + 1) * T * M, (int64_t)p.n_omega) - 1;
       const double a = p.omega[first] * kInv2Pi, b = p.omega[last] * kInv2Pi;
       w_lo = (a * b <= 0.0) ? 0.0 : fmin(fabs(a), fabs(b));
       w_hi = fmax(fabs(a), fabs(b));
+      rec_sortable = p.omega_step > 0.0 && a >= 0.0;
+    }
+    // compaction: run r = k*W + wave holds the k-th omegas of the wave's lanes;
+    // a step keeps some omega of run r iff s_wmin[r] < thr_s.  The recurrence
+    // sweep needs the thread's runs in ascending |omega| (a non-negative,
+    // increasing grid): then the runs a step keeps are a prefix k < K.
+    if (kCompact) {
+      __syncthreads();  // an earlier tile of this block is done with s_wmin, s_rank
+#pragma unroll
+      for (int k = 0; k < M; k++) {
+        double v = valid[k] ? fabs(om_rev[k]) : INFINITY;
+        for (int o = kWave / 2; o > 0; o >>= 1) v = fmin(v, __shfl_xor(v, o));
+        if (lane == 0) s_wmin[k * W + wv] = v;
+      }
+      __syncthreads();
+      if (tid < W * M) {
+        const double mine = s_wmin[tid];
+        int rk = 0;
+        for (int r = 0; r < W * M; r++) {
+          const double o = s_wmin[r];
+          rk += (o < mine || (o == mine && r < tid)) ? 1 : 0;
+        }
+        s_rank[tid] = rk;
+      }
+      // published by the barrier that opens the first step tile
     }
     double *dst[kMaxOut];
 #pragma unroll
@@ -312,19 +364,22 @@ k_spectrum(const KParams p) {
         }
       };
 
-      // one step of the recurrence sweep: the base phase in FP64, then each of
-      // the thread's omegas by rotation.  MASKED: the step is skipped when the
-      // whole block is above its threshold and each omega is masked otherwise;
-      // the unmasked form runs for tiles where no step cuts the block's range.
-      auto rec_step = [&](auto masked, double tau, double thr, float er, float ei, const float *a) {
-        constexpr bool kMasked = decltype(masked)::value;
-        if (kMasked && thr <= w_lo) return;        // every omega of the block filtered out
+      // one step of the recurrence sweep for the thread's first K omegas: the
+      // base phase in FP64, then each omega by rotation.  MASK 0: no masking
+      // (tiles where no step cuts the block's omega range); 1: the step is
+      // skipped when the whole block is above its threshold and each omega is
+      // masked otherwise; 2: only the K-th omega is masked (sorted tiles, where
+      // the runs below the step's last kept run keep all their omegas).
+      auto rec_step = [&](auto Kc, auto Mc, double tau, double thr, float er, float ei, const float *a) {
+        constexpr int K = decltype(Kc)::value;
+        constexpr int MASK = decltype(Mc)::value;
+        if (MASK == 1 && thr <= w_lo) return;        // every omega of the block filtered out
         float sn, cn;
         sincos_rev(om0_rev * tau, &sn, &cn);
 #pragma unroll
-        for (int k = 0; k < M; k++) {
+        for (int k = 0; k < K; k++) {
           float cm = cn, sm = sn;
-          if (kMasked) {
+          if (MASK == 1 || (MASK == 2 && k == K - 1)) {
             const bool act = fabs(om_rev[k]) < thr;
             cm = act ? cn : 0.f;
             sm = act ? sn : 0.f;
@@ -335,7 +390,7 @@ k_spectrum(const KParams p) {
             ps[k][w] = __builtin_fmaf(a[c], cm, ps[k][w]);
             ps[k][w + 3] = __builtin_fmaf(a[c], sm, ps[k][w + 3]);
           }
-          if (k + 1 < M) {
+          if (k + 1 < K) {
             const float cn2 = __builtin_fmaf(cn, er, -sn * ei);
             sn = __builtin_fmaf(cn, ei, sn * er);
             cn = cn2;
@@ -344,13 +399,13 @@ k_spectrum(const KParams p) {
       };
       // steps [j, jend) of the staged tile, single steps up to a multiple of 4,
       // then groups of 4 fetched with 16-byte LDS reads, then the tail
-      auto rec_sweep = [&](auto masked, int &j, int jend) {
+      auto rec_sweep = [&](auto Kc, auto Mc, int &j, int jend) {
         const int jg = (SR_REC_GROUP == 4) ? min(jend, (j + 3) & ~3) : jend;
         for (; j < jg; j++) {
           float a[NA];
 #pragma unroll
           for (int c = 0; c < NA; c++) a[c] = (float)s_amp[c][j];
-          rec_step(masked, s_tau[j], s_thr[j], s_rot[0][j], s_rot[1][j], a);
+          rec_step(Kc, Mc, s_tau[j], s_thr[j], s_rot[0][j], s_rot[1][j], a);
         }
         for (; j + 4 <= jend; j += 4) {
           const double2 t01 = *(const double2 *)&s_tau[j];
@@ -365,29 +420,98 @@ k_spectrum(const KParams p) {
           float a[NA];
 #pragma unroll
           for (int c = 0; c < NA; c++) a[c] = av[c].x;
-          rec_step(masked, t01.x, h01.x, er.x, ei.x, a);
+          rec_step(Kc, Mc, t01.x, h01.x, er.x, ei.x, a);
 #pragma unroll
           for (int c = 0; c < NA; c++) a[c] = av[c].y;
-          rec_step(masked, t01.y, h01.y, er.y, ei.y, a);
+          rec_step(Kc, Mc, t01.y, h01.y, er.y, ei.y, a);
 #pragma unroll
           for (int c = 0; c < NA; c++) a[c] = av[c].z;
-          rec_step(masked, t23.x, h23.x, er.z, ei.z, a);
+          rec_step(Kc, Mc, t23.x, h23.x, er.z, ei.z, a);
 #pragma unroll
           for (int c = 0; c < NA; c++) a[c] = av[c].w;
-          rec_step(masked, t23.y, h23.y, er.w, ei.w, a);
+          rec_step(Kc, Mc, t23.y, h23.y, er.w, ei.w, a);
         }
         for (; j < jend; j++) {
           float a[NA];
 #pragma unroll
           for (int c = 0; c < NA; c++) a[c] = (float)s_amp[c][j];
-          rec_step(masked, s_tau[j], s_thr[j], s_rot[0][j], s_rot[1][j], a);
+          rec_step(Kc, Mc, s_tau[j], s_thr[j], s_rot[0][j], s_rot[1][j], a);
         }
+      };
+
+      // one staged step of the direct sweeps for the omegas k with gate(k)
+      auto direct_step = [&](int j, auto gate) {
+        const double tau = s_tau[j];
+        const double thr = kSplit ? s_thr[j] : 0.0;
+        Amp a[NA];
+#pragma unroll
+        for (int c = 0; c < NA; c++) a[c] = s_amp[c][j];
+#pragma unroll
+        for (int k = 0; k < M; k++) {
+          if (!gate(k)) continue;
+          // FP64: phase in radians and the reference's exact test
+          // |phase_s - phase_{s-1}| < pi.  Split: phase in revolutions
+          // (omega/2pi precomputed) and the same inequality in its
+          // threshold form |omega|/2pi < thr_s, staged per step; the two
+          // can disagree only within rounding of the boundary.
+          const double ph = (kSplit ? om_rev[k] : om[k]) * tau;
+          bool act;
+          if (kSplit) {
+            act = valid[k] && (fabs(om_rev[k]) < thr);
+          } else {
+            const double d = ph - prev[k];
+            prev[k] = ph;
+            act = valid[k] && (fabs(d) < kPi);
+          }
+          if (act) {
+            if (kSplit) {
+              float sn, cn;
+              sincos_rev(ph, &sn, &cn);
+              // word layout shared with the FP64 path: per amplitude vector
+              // v (far: A; near: v1 then v2) [v cos | v sin], 3 words each
+#pragma unroll
+              for (int c = 0; c < NA; c++) {
+                const int w = 6 * (c / 3) + (c % 3);
+                ps[k][w] = __builtin_fmaf((float)a[c], cn, ps[k][w]);
+                ps[k][w + 3] = __builtin_fmaf((float)a[c], sn, ps[k][w + 3]);
+              }
+            } else {
+              double sn, cn;
+              sincos_reduced(ph, &sn, &cn);
+#pragma unroll
+              for (int c = 0; c < NA; c++) {
+                const int w = 6 * (c / 3) + (c % 3);
+                fs[k][w] = fma((double)a[c], cn, fs[k][w]);
+                fs[k][w + 3] = fma((double)a[c], sn, fs[k][w + 3]);
+              }
+            }
+          }
+        }
+      };
+      // split: Kahan fold of the segment's partial sums into the LDS S, C
+      auto kahan_fold = [&]() {
+        if (!kSplit) return;
+#pragma unroll
+        for (int k = 0; k < M; k++)
+#pragma unroll
+          for (int c = 0; c < NC; c++) {
+            float &S = s_ks[(c * M + k) * T];
+            float &C = s_kc[(c * M + k) * T];
+            const float yv = ps[k][c] - C;
+            const float tv = S + yv;
+            C = (tv - S) - yv;
+            S = tv;
+            ps[k][c] = 0.f;
+          }
       };
 
       for (int64_t t0 = 0; t0 < n_act; t0 += T) {
         const int cnt = uni((int)min((int64_t)T, n_act - t0));
         __syncthreads();  // previous tile fully consumed
         int step_on = 1;
+        Amp st_amp[NA];
+        double st_tau = 0.0, st_thr = 0.0;
+        float st_rot[2] = {1.f, 0.f};
         if (tid < cnt) {
           // ---- stage: omega-independent part of step s = t0 + tid, in FP64 ----
           const int64_t s = t0 + tid;
@@ -424,9 +548,9 @@ k_spectrum(const KParams p) {
               const double r2 = phx * Ax + phy * Ay + 0.0 * Az;
               Ax = r0; Ay = r1; Az = r2;
             }
-            s_amp[0][tid] = (Amp)Ax;
-            s_amp[1][tid] = (Amp)Ay;
-            s_amp[2][tid] = (Amp)Az;
+            st_amp[0] = (Amp)Ax;
+            st_amp[1] = (Amp)Ay;
+            st_amp[2] = (Amp)Az;
           } else {
             // kernel_nearfield.cl:67-88; omega of c1 is applied at the record
             const double rx = gx - xs, ry = gy - ys, rz = gz - zs;
@@ -440,113 +564,125 @@ k_spectrum(const KParams p) {
             const double rInv = 1.0 / R;
             const double nx = rInv * rx, ny = rInv * ry, nz = rInv * rz;
             const double r2 = rInv * rInv;
-            s_amp[0][tid] = (Amp)(rInv * (b0x - nx));
-            s_amp[1][tid] = (Amp)(rInv * (b0y - ny));
-            s_amp[2][tid] = (Amp)(rInv * (b0z - nz));
-            s_amp[3][tid] = (Amp)(r2 * nx);
-            s_amp[4][tid] = (Amp)(r2 * ny);
-            s_amp[5][tid] = (Amp)(r2 * nz);
+            st_amp[0] = (Amp)(rInv * (b0x - nx));
+            st_amp[1] = (Amp)(rInv * (b0y - ny));
+            st_amp[2] = (Amp)(rInv * (b0z - nz));
+            st_amp[3] = (Amp)(r2 * nx);
+            st_amp[4] = (Amp)(r2 * ny);
+            st_amp[5] = (Amp)(r2 * nz);
           }
-          s_tau[tid] = tau;
+          st_tau = tau;
           if (kSplit) {
             // |omega tau_s - omega tau_{s-1}| < pi  <=>  |omega|/2pi < (1/2)/|tau_s - tau_{s-1}|
             // (phase_{-1} = 0 at the first step); 1/0 gives +inf: always active
             const double thr = 0.5 / fabs(tau - tau_prev);
-            s_thr[tid] = thr;
+            st_thr = thr;
             step_on = thr > w_hi;
           }
-          if (kRec) {
-            float sr, cr;
-            sincos_rev(rot_rev * tau, &sr, &cr);
-            s_rot[0][tid] = cr;
-            s_rot[1][tid] = sr;
+          if (kRec) sincos_rev(rot_rev * tau, &st_rot[1], &st_rot[0]);
+        }
+        // compaction (tiles without a snapshot record): bucket_s = number of runs
+        // that keep some omega at step s; a stable counting sort puts the steps
+        // in descending bucket order, so run r of rank rho sweeps exactly the
+        // first s_cntb[rho] steps.  Steps of bucket 0 are needed by no run.
+        const bool sort_tile = kCompact && (!kRec || rec_sortable) &&
+                               uni((cur.next >= 0 && cur.next < t0 + cnt) ? 0 : 1);
+        int pos = tid;
+        if (sort_tile) {
+          const int R = W * M;
+          int bucket = 0;
+          if (tid < cnt)
+            for (int r = 0; r < R; r++) bucket += (s_wmin[r] < st_thr) ? 1 : 0;
+          // per-wave counts in the order (bucket R..1) x (wave 0..W-1)
+          uint64_t mine = 0;
+          for (int b = 1; b <= R; b++) {
+            const uint64_t m = __ballot(bucket == b);
+            if (bucket == b) mine = m;
+            if (lane == 0) s_hist[(R - b) * W + wv] = __popcll(m);
           }
+          __syncthreads();
+          if (wv == 0) {
+            const int n_ent = R * W;   // <= 64: one entry per lane
+            const int v = lane < n_ent ? s_hist[lane] : 0;
+            int inc = v;
+            for (int o = 1; o < kWave; o <<= 1) {
+              const int y = __shfl_up(inc, o);
+              if (lane >= o) inc += y;
+            }
+            if (lane < n_ent) s_off[lane] = inc - v;
+            // steps with bucket > b = everything before entry (b, wave 0)
+            if (lane < n_ent && lane % W == 0) s_cntb[R - lane / W] = inc - v;
+            if (lane == n_ent - 1) s_cntb[0] = inc;
+          }
+          __syncthreads();
+          const uint64_t below = (lane == 0) ? 0ull : (~0ull >> (kWave - lane));
+          pos = (bucket > 0) ? s_off[(R - bucket) * W + wv] + __popcll(mine & below) : -1;
+        }
+        if (tid < cnt && pos >= 0) {
+          s_tau[pos] = st_tau;
+          if (kSplit) s_thr[pos] = st_thr;
+          if (kRec) {
+            s_rot[0][pos] = st_rot[0];
+            s_rot[1][pos] = st_rot[1];
+          }
+#pragma unroll
+          for (int c = 0; c < NA; c++) s_amp[c][pos] = st_amp[c];
         }
         // rec: the tile needs no masking when every step keeps the whole block
         const bool tile_on = kRec ? (__syncthreads_and(step_on) != 0) : (__syncthreads(), false);
 
         // ---- sweep, split at snapshot records falling inside the tile ----
-        int j = 0;
-        while (j < cnt) {
-          const bool rec_here = (cur.next >= 0) && (cur.next < t0 + cnt);
-          // records are strictly increasing, so cur.next >= t0 + j; the max()
-          // only guarantees forward progress
-          const int jend = uni(rec_here ? max(j + 1, (int)(cur.next - t0) + 1) : cnt);
+        if (sort_tile) {
+          // run k of this wave (rank s_rank[k*W + wave]) needs the first ck[k] steps
+          int ck[M];
+          int jmax = 0;
+#pragma unroll
+          for (int k = 0; k < M; k++) {
+            ck[k] = uni(s_cntb[s_rank[k * W + wv]]);
+            jmax = max(jmax, ck[k]);
+          }
           if (kRec) {
-            if (tile_on)
-              rec_sweep(std::false_type(), j, jend);
-            else
-              rec_sweep(std::true_type(), j, jend);
+            // ascending runs: ck[0] >= ck[1] >= ..., so steps [ck[K], ck[K-1])
+            // keep exactly the runs k < K (ck[M] = 0)
+            int j = 0;
+            if (tile_on) {
+              // every valid omega of the block kept at every step: the wave's
+              // valid runs all have ck = cnt, runs past n_omega have ck = 0
+              rec_sweep(IC<M>(), IC<0>(), j, ck[0]);
+            } else {
+              rec_sweep(IC<M>(), IC<2>(), j, ck[M - 1]);
+              if (M >= 2) rec_sweep(IC<(M >= 2 ? M - 1 : 1)>(), IC<2>(), j, ck[M >= 2 ? M - 2 : 0]);
+              if (M >= 3) rec_sweep(IC<(M >= 3 ? M - 2 : 1)>(), IC<2>(), j, ck[M >= 3 ? M - 3 : 0]);
+              if (M >= 4) rec_sweep(IC<(M >= 4 ? M - 3 : 1)>(), IC<2>(), j, ck[0]);
+            }
           } else {
 #pragma unroll 2
-            for (; j < jend; j++) {
-              const double tau = s_tau[j];
-              const double thr = kSplit ? s_thr[j] : 0.0;
-              Amp a[NA];
-#pragma unroll
-              for (int c = 0; c < NA; c++) a[c] = s_amp[c][j];
-#pragma unroll
-              for (int k = 0; k < M; k++) {
-                // FP64: phase in radians and the reference's exact test
-                // |phase_s - phase_{s-1}| < pi.  Split: phase in revolutions
-                // (omega/2pi precomputed) and the same inequality in its
-                // threshold form |omega|/2pi < thr_s, staged per step; the two
-                // can disagree only within rounding of the boundary.
-                const double ph = (kSplit ? om_rev[k] : om[k]) * tau;
-                bool act;
-                if (kSplit) {
-                  act = valid[k] && (fabs(om_rev[k]) < thr);
-                } else {
-                  const double d = ph - prev[k];
-                  prev[k] = ph;
-                  act = valid[k] && (fabs(d) < kPi);
-                }
-                if (act) {
-                  if (kSplit) {
-                    float sn, cn;
-                    sincos_rev(ph, &sn, &cn);
-                    // word layout shared with the FP64 path: per amplitude vector
-                    // v (far: A; near: v1 then v2) [v cos | v sin], 3 words each
-#pragma unroll
-                    for (int c = 0; c < NA; c++) {
-                      const int w = 6 * (c / 3) + (c % 3);
-                      ps[k][w] = __builtin_fmaf((float)a[c], cn, ps[k][w]);
-                      ps[k][w + 3] = __builtin_fmaf((float)a[c], sn, ps[k][w + 3]);
-                    }
-                  } else {
-                    double sn, cn;
-                    sincos_reduced(ph, &sn, &cn);
-#pragma unroll
-                    for (int c = 0; c < NA; c++) {
-                      const int w = 6 * (c / 3) + (c % 3);
-                      fs[k][w] = fma((double)a[c], cn, fs[k][w]);
-                      fs[k][w + 3] = fma((double)a[c], sn, fs[k][w + 3]);
-                    }
-                  }
-                }
-              }
+            for (int j = 0; j < jmax; j++) direct_step(j, [&](int k) { return j < ck[k]; });
+          }
+          kahan_fold();
+        } else {
+          int j = 0;
+          while (j < cnt) {
+            const bool rec_here = (cur.next >= 0) && (cur.next < t0 + cnt);
+            // records are strictly increasing, so cur.next >= t0 + j; the max()
+            // only guarantees forward progress
+            const int jend = uni(rec_here ? max(j + 1, (int)(cur.next - t0) + 1) : cnt);
+            if (kRec) {
+              if (tile_on)
+                rec_sweep(IC<M>(), IC<0>(), j, jend);
+              else
+                rec_sweep(IC<M>(), IC<1>(), j, jend);
+            } else {
+#pragma unroll 2
+              for (; j < jend; j++) direct_step(j, [](int) { return true; });
             }
-          }
-          if (kSplit) {
-            // Kahan fold of the segment's partial sums into the LDS S, C
-#pragma unroll
-            for (int k = 0; k < M; k++)
-#pragma unroll
-              for (int c = 0; c < NC; c++) {
-                float &S = s_ks[(c * M + k) * T];
-                float &C = s_kc[(c * M + k) * T];
-                const float yv = ps[k][c] - C;
-                const float tv = S + yv;
-                C = (tv - S) - yv;
-                S = tv;
-                ps[k][c] = 0.f;
-              }
-          }
-          if (rec_here) {
-            emit(cur.slot);
-            const int64_t last = cur.next;
-            cur.slot++;
-            cur.advance(pt, p.n_snaps, last);
+            kahan_fold();
+            if (rec_here) {
+              emit(cur.slot);
+              const int64_t last = cur.next;
+              cur.slot++;
+              cur.advance(pt, p.n_snaps, last);
+            }
           }
         }
       }

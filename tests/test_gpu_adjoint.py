@@ -74,10 +74,13 @@ def test_adjoint_decomposition_does_not_change_the_result(groups, waves):
     tr, axes, dt, S0, T, W, G = adjoint_case("total")
     a, aw = gpu_grads(tr, axes, dt, G, "total", "double", engine.Tuning(slices=groups, waves_per_block=waves))
     b, bw = gpu_grads(tr, axes, dt, G, "total", "double")
+    # other direction groups regroup the FP64 sums, and the gradient sums
+    # cancel strongly (terms ~1e2 x the result): agreement to ~1e-13 is all
+    # that FP64 rounding allows, so the bar is 1e-11
     for p in range(len(a)):
         for c in range(6):
-            assert rel_l2(a[p][c], b[p][c]) < 1e-13
-    assert rel_l2(aw, bw) < 1e-13
+            assert rel_l2(a[p][c], b[p][c]) < 1e-11
+    assert rel_l2(aw, bw) < 1e-11
 
 
 def test_autograd_function_and_finite_differences():
@@ -159,3 +162,187 @@ def test_adjoint_betatron_full_grid_f64_vs_split():
         want = np.concatenate([r[c] for r in ref])
         assert rel_l2(g64[c].cpu().numpy(), want) < 1e-10
     assert rel_l2(g64[6].cpu().numpy(), ref_w) < 1e-10
+
+
+def test_adjoint_config5_fd_50_coordinates():
+    """The adjoint workload at full size (256 e-, 4000 fp32-rounded steps,
+    128 x 32 x 32 over 1-75 keV x 40 mrad); loss = sum W (S - T)^2 with T from
+    tracks pushed with 1% stronger focusing and W = sin(theta).
+
+    * loss.backward() through far_spectrum (split precision) agrees with the
+      FP64 adjoint to 1e-4 rel-L2, and its forward with the FP64 forward;
+    * the FP64 adjoint agrees with FP64 central finite differences of the loss
+      at 50 seeded coordinates (x, y, z, ux, uy, uz of a step, or a weight)
+      to 1e-6 relative, Richardson-extrapolated over a ladder of steps.
+
+    A weight's differences come from the GPU FP64 forward of the particle
+    with the weight moved.  A sample's position or momentum enters at most
+    two terms of the particle's sum: x_s only tau_s = t_s - n.x_s (linear in
+    x) and the filter tests of steps s and s+1; u_s the amplitudes of steps
+    s-1 and s.  Moving them through the whole FP64 forward cannot resolve
+    their derivatives at this scale: tau carries ~5e-20 m of rounding
+    (|t| ~ 3e-4 m) while any position step above ~1e-15 m flips a filter
+    decision somewhere on the grid and makes L jump (the adjoint
+    differentiates the piece on which the mask is fixed), and the 4000-term
+    sums bury a momentum's effect in their rounding (1e-7 .. 5e-6 measured).
+    So those differences are formed with the mask held: the particle's FP64
+    complex field (GPU 'cartesian_complex' forward) with the affected terms
+    replaced by their values at the moved coordinate (the phase moved
+    exactly by -+ omega n_c h for a position; momentum-moved amplitudes in
+    extended precision), the change of |F|^2 written out so that nothing
+    large cancels."""
+    wl = workloads.config5()
+    tg = workloads.config5(target=True)
+    hb, ax, dt = wl.batch, wl.axes, wl.batch.dt
+    grid = engine.DeviceGrid("far", ax["omega"], ax["theta"], ax["phi"])
+    dev = grid.device
+    S64 = engine.spectrum(engine.DeviceBatch(hb, dev), grid, comp="total", precision="double")[0]
+    T = engine.spectrum(engine.DeviceBatch(tg.batch, dev), grid, comp="total", precision="double")[0]
+    Wt = torch.sin(torch.as_tensor(ax["theta"], device=dev))[None, None, :, None].expand_as(S64).contiguous()
+
+    cols = ("x", "y", "z", "ux", "uy", "uz", "weight")
+    leaves = [torch.tensor(getattr(hb, c), device=dev, requires_grad=True) for c in cols]
+    S = autograd.far_spectrum(*leaves, hb, grid, comp="total", precision="float")[0]
+    r = float(torch.linalg.norm(S.detach() - S64) / torch.linalg.norm(S64))
+    print(f"config5 split forward vs FP64 rel-L2 {r:.3e}", flush=True)
+    assert r < 1e-4
+    torch.sum(Wt * (S - T) ** 2).backward()
+    G = 2 * Wt * (S64 - T)
+    g64 = engine.far_adjoint(engine.DeviceBatch(hb, dev), grid, [G], precision="double")
+    torch.cuda.synchronize()
+    for c in range(7):
+        r = float(torch.linalg.norm(leaves[c].grad - g64[c]) / torch.linalg.norm(g64[c]))
+        print(f"config5 split adjoint vs FP64 adjoint, {cols[c]}: rel-L2 {r:.3e}", flush=True)
+        assert r < 1e-4
+    g64 = [g.cpu().numpy() for g in g64]
+
+    # grid on the host exactly as the device holds it, in the device order (phi, theta, omega)
+    om = grid.omega.cpu().numpy()[None, None, :]
+    st, ct = grid.sin_theta.cpu().numpy()[None, :, None], grid.cos_theta.cpu().numpy()[None, :, None]
+    sp, cp = grid.sin_phi.cpu().numpy()[:, None, None], grid.cos_phi.cpu().numpy()[:, None, None]
+    nvec = (st * cp, st * sp, ct * np.ones_like(cp))
+    rng = np.random.default_rng(50)
+    off = hb.offsets
+    tl = wl.track_list()
+    single = {}
+
+    def own(p):
+        if p not in single:
+            b = engine.DeviceBatch(engine.pack_tracks([tl[p]], dt=dt), dev)
+            Sp = engine.spectrum(b, grid, comp="total", precision="double")[0]
+            cc = engine.spectrum(b, grid, comp="cartesian_complex", precision="double")
+            scale = np.sqrt(float(hb.weight[p])) * dt
+            F = [torch.complex(cc[2 * c], cc[2 * c + 1]).cpu().numpy()[0] / scale for c in range(3)]
+            single[p] = (b, Sp, F)
+        return single[p]
+
+    def phase(p, k):
+        """FP64 phase of sample k of particle p, formed as the kernels form it."""
+        a = int(off[p]) + k
+        t = float(np.uint32(int(hb.it_start[p]) + k)) * dt
+        return om * (t - (hb.x[a] * nvec[0] + hb.y[a] * nvec[1] + hb.z[a] * nvec[2]))
+
+    def mom(p, k):
+        a = int(off[p]) + k
+        return np.array([hb.ux[a], hb.uy[a], hb.uz[a]])
+
+    LD = np.longdouble
+    nvl = [a.astype(LD) for a in nvec]
+
+    def amp(u0, u1):
+        """A(n) of a step from its two momenta (synchrad/kernel_farfield.cl:76-94),
+        in extended precision: 1 - n.bbar ~ 3e-6 loses 5 digits to cancellation,
+        which would otherwise swamp the momentum differences."""
+        u0, u1, one = u0.astype(LD), u1.astype(LD), LD(1)
+        b0 = u0 * (one / np.sqrt(one + (u0[0] * u0[0] + u0[1] * u0[1] + u0[2] * u0[2])))
+        b1 = u1 * (one / np.sqrt(one + (u1[0] * u1[0] + u1[1] * u1[1] + u1[2] * u1[2])))
+        acc = (b1 - b0) * (one / LD(dt))
+        mb = LD(0.5) * (b1 + b0)
+        c2 = one / (one - (mb[0] * nvl[0] + mb[1] * nvl[1] + mb[2] * nvl[2]))
+        c1 = (acc[0] * nvl[0] + acc[1] * nvl[1] + acc[2] * nvl[2]) * c2 * c2
+        return [c1 * (nvl[c] - mb[c]) - c2 * acc[c] for c in range(3)]
+
+    def expo(p, s):
+        """mask_s exp(i phase_s): the filter decision of the unperturbed track."""
+        ph = phase(p, s)
+        prev = phase(p, s - 1) if s > 0 else 0.0
+        return np.where(np.abs(ph - prev) < np.pi, np.exp(1j * ph), 0.0)
+
+    worst = 0.0
+    for i in range(50):
+        p = int(rng.integers(hb.n_particles))
+        c = int(rng.integers(7))
+        n_p = int(off[p + 1] - off[p])
+        s = 0 if c == 6 else int(rng.integers(n_p - 1 if c < 3 else n_p))
+        b, Sp0, F = own(p)
+        rest2 = 2 * (S64 - Sp0 - T)
+        v = float(getattr(hb, cols[c])[int(off[p]) + s]) if c < 6 else float(hb.weight[p])
+        wdt2 = float(hb.weight[p]) * dt * dt
+        if c < 6:
+            Wn = Wt[0].cpu().numpy()
+            restn = rest2[0].cpu().numpy()
+            F2 = sum(np.abs(F[k]) ** 2 for k in range(3))
+        if c < 3:
+            t0 = [A.astype(np.float64) * expo(p, s) for A in amp(mom(p, s), mom(p, s + 1))]
+            FbT = sum(np.conj(F[k]) * t0[k] for k in range(3))
+            T2 = sum(np.abs(t0[k]) ** 2 for k in range(3))
+            h0 = 1e-3 / (om.max() * np.abs(nvec[c]).max())
+
+            def D(h):
+                # S_p(+-h) = w dt^2 |F + t0 (exp(-+i d) - 1)|^2, d = omega n_c h
+                d = om * nvec[c] * h
+                dS = 4.0 * wdt2 * np.sin(d) * FbT.imag
+                q = np.sin(0.5 * d) ** 2
+                Ssum = wdt2 * (2.0 * F2 - 8.0 * q * FbT.real + 8.0 * q * T2)
+                return float(np.sum(Wn * dS * (Ssum + restn))) / (2 * h)
+        elif c < 6:
+            # u_s enters the amplitudes of steps s-1 and s (those that exist)
+            steps = [q for q in (s - 1, s) if 0 <= q <= n_p - 2]
+            ex = {q: expo(p, q) for q in steps}
+            h0 = 1e-3 * max(abs(v), 1.0)
+
+            def D(h):
+                he = (v + h) - v
+                diff = [0.0, 0.0, 0.0]      # d(+h) - d(-h)
+                summ = [0.0, 0.0, 0.0]      # d(+h) + d(-h)
+                for q in steps:
+                    us = [mom(p, q), mom(p, q + 1)]
+                    j = 0 if q == s else 1
+                    up = [u.copy() for u in us]
+                    um = [u.copy() for u in us]
+                    up[j][c - 3] = v + he
+                    um[j][c - 3] = v - he
+                    A0, Ap, Am = amp(*us), amp(*up), amp(*um)
+                    for k in range(3):
+                        diff[k] = diff[k] + (Ap[k] - Am[k]).astype(np.float64) * ex[q]
+                        summ[k] = summ[k] + (Ap[k] + Am[k] - 2 * A0[k]).astype(np.float64) * ex[q]
+                dS = wdt2 * sum(2.0 * (np.conj(F[k]) * diff[k]).real + (diff[k] * np.conj(summ[k])).real
+                                for k in range(3))
+                dp2 = sum(np.abs(0.5 * (summ[k] + diff[k])) ** 2 for k in range(3))
+                dm2 = sum(np.abs(0.5 * (summ[k] - diff[k])) ** 2 for k in range(3))
+                Ssum = wdt2 * (2.0 * F2 + sum(2.0 * (np.conj(F[k]) * summ[k]).real for k in range(3)) + dp2 + dm2)
+                return float(np.sum(Wn * dS * (Ssum + restn))) / (2 * he)
+        else:
+            arr = b.weight
+            h0 = 1e-3 * abs(v)
+
+            def D(h):
+                he = (v + h) - v
+                arr[0] = v + he
+                Spp = engine.spectrum(b, grid, comp="total", precision="double")[0]
+                arr[0] = v - he
+                Spm = engine.spectrum(b, grid, comp="total", precision="double")[0]
+                arr[0] = v
+                return float(torch.sum(Wt * (Spp - Spm) * (Spp + Spm + rest2))) / (2 * he)
+
+        Ds = [D(h0 * 0.25 ** k) for k in range(6)]
+        R = [(16 * Ds[k + 1] - Ds[k]) / 15 for k in range(5)]
+        k = int(np.argmin([abs(R[j] - R[j + 1]) for j in range(4)]))
+        est = R[k]
+        got = float(g64[c][off[p] + s]) if c < 6 else float(g64[6][p])
+        err = abs(est - got) / abs(got) if got != 0.0 else abs(est)
+        worst = max(worst, err)
+        print(f"config5 FD #{i:2d} particle {p:3d} {cols[c]:6s} step {s:4d}: adjoint {got:+.10e} "
+              f"FD {est:+.10e} rel {err:.2e} (h0 {h0:.3e})", flush=True)
+        assert err < 1e-6, (i, p, c, s, est, got)
+    print(f"config5 FP64 adjoint vs FP64 central FD, 50 coordinates: worst {worst:.2e}", flush=True)
