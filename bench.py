@@ -62,6 +62,8 @@ def parse():
     ap.add_argument("--per-particle-share", type=int, default=400,
                     help="config 3: particles timed with one launch each")
     ap.add_argument("--copy-calibration", type=int, default=0)
+    ap.add_argument("--active-particles", type=int, default=100,
+                    help="particles over which the filter active fraction is counted")
     return ap.parse_args()
 
 
@@ -117,6 +119,35 @@ def rel_l2_dev(a_list, b_list):
     num = sum(float(torch.sum((a - b) ** 2)) for a, b in zip(a_list, b_list))
     den = sum(float(torch.sum(b ** 2)) for b in b_list)
     return (num / den) ** 0.5 if den > 0 else float("nan")
+
+
+def active_fraction(wl, n_particles):
+    """Share of point-step evaluations that pass the reference's filter
+    |omega (tau_s - tau_{s-1})| < pi (far field, phase_{-1} = 0), counted on
+    the device for the first `n_particles` particles in the threshold form
+    |omega| < pi / |tau_s - tau_{s-1}|."""
+    import torch
+    if wl.mode != "far":
+        return None
+    hb, ax = wl.batch, wl.axes
+    dev = torch.device("cuda")
+    om = torch.sort(torch.abs(torch.as_tensor(2 * np.pi * ax["omega"], device=dev)))[0]
+    th, ph = np.meshgrid(ax["theta"], ax["phi"])
+    n = torch.as_tensor(np.stack([np.sin(th) * np.cos(ph), np.sin(th) * np.sin(ph), np.cos(th)]).reshape(3, -1),
+                        device=dev)
+    act = tot = 0
+    for p in range(min(n_particles, hb.n_particles)):
+        a, b = int(hb.offsets[p]), int(hb.offsets[p + 1])
+        if b - a < 2:
+            continue
+        X = torch.as_tensor(np.stack([hb.x[a:b - 1], hb.y[a:b - 1], hb.z[a:b - 1]]), device=dev)
+        t = (int(hb.it_start[p]) + torch.arange(b - a - 1, device=dev, dtype=torch.float64)) * hb.dt
+        tau = t[:, None] - X.T @ n
+        d = torch.abs(torch.diff(tau, dim=0, prepend=torch.zeros_like(tau[:1])))
+        thr = torch.where(d > 0, np.pi / d, torch.full_like(d, float("inf")))
+        act += int(torch.searchsorted(om, thr.reshape(-1).contiguous()).sum())
+        tot += thr.numel() * om.numel()
+    return act / tot if tot else None
 
 
 def traffic_from_profile():
@@ -188,6 +219,9 @@ def bench_forward(args):
     log(f"{main_prec}: {ms:.1f} ms/spectrum ({ms_kernel:.1f} ms in the library call, {wall:.1f} ms wall), "
         f"{value:.1f} G evals/s, {achieved_tf:.1f} TFLOP/s algorithmic")
     nmax = int(np.diff(wl.batch.offsets).max())
+    f_act = active_fraction(wl, args.active_particles)
+    if f_act is not None:
+        log(f"filter active fraction {f_act:.4f} (first {args.active_particles} particles)")
     res = {
         "metric": METRIC,
         "value": round(value, 3),
@@ -211,9 +245,13 @@ def bench_forward(args):
                      "unit": "TFLOP/s", "frac": round(achieved_tf / peak, 4),
                      "traffic": traffic_from_profile() if cfg == 2 else None,
                      "kernel_ms": round(ms_kernel, 3),
+                     "filter_active_fraction": round(f_act, 4) if f_act is not None else None,
+                     "frac_filter_aware": (round(evals * (9.0 + 40.0 * f_act) / (ms_kernel * 1e-3) / 1e12 / peak, 4)
+                                           if (f_act is not None and flop == 49.0) else None),
                      "basis": f"{flop:g} algorithmic flop per point-step evaluation x "
                               f"{evals:.4g} evaluations per launch / mean duration of the library "
-                              f"call (HIP events on its stream)"},
+                              f"call (HIP events on its stream); frac_filter_aware counts 9 flop "
+                              f"for an evaluation the filter rejects and 49 for one it keeps"},
     }
 
     if main_prec == "float" and not args.no_f64:

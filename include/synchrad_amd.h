@@ -55,13 +55,15 @@ enum {
 enum { SR_PREC_F64 = 0, SR_PREC_SPLIT = 1 };
 
 /*
- * Sweep of the split path.  SR_SWEEP_DIRECT (what SR_SWEEP_AUTO selects) forms
- * every phase in FP64.  SR_SWEEP_RECURRENCE, for grids that declare a constant
- * omega_step, forms exp(i omega tau) in FP64/FP32 for one omega per thread and
- * advances it to the thread's other omegas by complex multiplication; the
- * |dphase| < pi filter is then applied as the equivalent threshold
- * omega < pi/|tau_s - tau_{s-1}|.  The FP64 path always sweeps directly and
- * filters exactly as the reference does.
+ * Sweep of the split path.  SR_SWEEP_DIRECT forms every phase in FP64.
+ * SR_SWEEP_RECURRENCE, for grids that declare a constant omega_step, forms
+ * exp(i omega tau) in FP64/FP32 for one omega per thread and advances it to the
+ * thread's other omegas by complex multiplication, and sweeps only the steps
+ * that keep some omega of each 64-omega run.  Both apply the |dphase| < pi
+ * filter as the equivalent threshold |omega| < pi/|tau_s - tau_{s-1}|.
+ * SR_SWEEP_AUTO selects the recurrence on linear grids and the direct sweep
+ * otherwise.  The FP64 path always sweeps directly and filters
+ * exactly as the reference does.
  */
 enum { SR_SWEEP_AUTO = 0, SR_SWEEP_DIRECT = 1, SR_SWEEP_RECURRENCE = 2 };
 

@@ -110,6 +110,9 @@ struct KParams {
   int32_t n_out;
   double *out[kMaxOut];
   double *scratch;
+  // 1: one snapshot and one output -- each thread sums its points over the
+  // block's particles in LDS and adds the sum to the output once
+  int32_t acc_lds;
   // tiling
   uint32_t n_chunks;
   uint32_t n_dir;
@@ -128,12 +131,24 @@ template <int MODE, int M>
 constexpr size_t kahan_lds_bytes(int T) {
   return (size_t)2 * (MODE == SR_MODE_FAR ? 6 : 12) * M * T * sizeof(float);
 }
+// Dynamic LDS after the Kahan state: the per-thread particle sums of the
+// single output when KParams::acc_lds is set, [M][T] doubles.
+template <int M>
+constexpr size_t acc_lds_bytes(int T) {
+  return (size_t)M * T * sizeof(double);
+}
 
 // Register budget per variant, as the minimum number of waves each SIMD must
 // be able to hold: the split sweeps keep their Kahan state in LDS, so their
 // registers hold the partial sums of M omegas plus one staged step group.
+// (far recurrence with M = 2: 4 waves, measured 3% faster than 3 despite a
+// few spilled stage registers; the other recurrence variants spill heavily
+// at 4 and run at 2)
 #ifndef SR_REC_WAVES
 #define SR_REC_WAVES 2
+#endif
+#ifndef SR_REC_WAVES_FAR2
+#define SR_REC_WAVES_FAR2 4
 #endif
 #ifndef SR_REC_GROUP
 #define SR_REC_GROUP 4
@@ -141,21 +156,29 @@ constexpr size_t kahan_lds_bytes(int T) {
 #ifndef SR_DIRECT_WAVES
 #define SR_DIRECT_WAVES 3
 #endif
-// Split direct sweep: the steps of a tile are put in an order in which every
+#ifndef SR_F64_WAVES
+#define SR_F64_WAVES 4
+#endif
+// Recurrence sweep: the steps of a tile are put in an order in which every
 // 64-omega run of the block finds the steps that keep any of its omegas first,
-// so a wave sweeps only those (a stable counting sort per tile).
+// so a wave sweeps only those (a stable counting sort per tile).  The direct
+// sweep measured slower with it (the sort's registers cost it a wave per SIMD
+// and skipped steps are already cheap there), so it keeps the tile order.
 #ifndef SR_COMPACT
 #define SR_COMPACT 1
 #endif
 
-template <int PREC, int SWEEP, int M>
+template <int MODE, int PREC, int SWEEP, int M>
 struct WavesPerEU {
-  static constexpr int value = (PREC == SR_PREC_F64) ? 3 : (SWEEP == SWEEP_REC ? SR_REC_WAVES : (M >= 4 ? 2 : SR_DIRECT_WAVES));
+  static constexpr int value =
+      (PREC == SR_PREC_F64) ? SR_F64_WAVES
+      : (SWEEP == SWEEP_REC) ? ((MODE == SR_MODE_FAR && M == 2) ? SR_REC_WAVES_FAR2 : SR_REC_WAVES)
+                             : (M >= 4 ? 2 : SR_DIRECT_WAVES);
 };
 
 template <int MODE, int PREC, int SWEEP, int M>
 __global__ void __launch_bounds__(kMaxT)
-__attribute__((amdgpu_waves_per_eu(WavesPerEU<PREC, SWEEP, M>::value)))
+__attribute__((amdgpu_waves_per_eu(WavesPerEU<MODE, PREC, SWEEP, M>::value)))
 k_spectrum(const KParams p) {
   static_assert(!(PREC == SR_PREC_F64 && SWEEP == SWEEP_REC), "recurrence sweep is split-only");
   static_assert(PREC == SR_PREC_SPLIT || M == 1, "the FP64 sweep holds one omega per thread");
@@ -164,7 +187,7 @@ k_spectrum(const KParams p) {
   constexpr bool kRec = (SWEEP == SWEEP_REC);
   constexpr int NA = (MODE == SR_MODE_FAR) ? 3 : 6;   // amplitude words per step
   constexpr int NC = 2 * NA;                           // real accumulators per omega
-  constexpr bool kCompact = kSplit && (SR_COMPACT != 0);
+  constexpr bool kCompact = kSplit && kRec && (SR_COMPACT != 0);
   constexpr int kMaxW = kMaxT / kWave;                 // waves per block
   constexpr int kMaxRuns = kMaxW * 4;                  // 64-omega runs per block (M <= 4)
 
@@ -188,6 +211,8 @@ k_spectrum(const KParams p) {
   const int lane = tid % kWave;
   float *const s_ks = s_dyn + tid;                       // S[c*M+k] at s_ks[(c*M+k)*T]
   float *const s_kc = s_dyn + (size_t)NC * M * T + tid;  // C likewise
+  // particle sums of the single output, k-th omega at s_acc[k*T]
+  double *const s_acc = (double *)((char *)s_dyn + (kSplit ? kahan_lds_bytes<MODE, M>(T) : 0)) + tid;
   const double dt = p.dt;
   const double dtInv = 1.0 / dt;
   const bool spheric = (p.comp == SR_COMP_SPHERIC || p.comp == SR_COMP_SPHERIC_COMPLEX);
@@ -277,6 +302,10 @@ k_spectrum(const KParams p) {
                                    : p.out[c])
                    : nullptr;
 
+    if (p.acc_lds) {
+#pragma unroll
+      for (int k = 0; k < M; k++) s_acc[k * T] = 0.0;
+    }
     const int64_t p_lo = p.slice_lo[q], p_hi = p.slice_lo[q + 1];
     for (int64_t ip = p_lo; ip < p_hi; ip++) {
       const Particle pt = load_particle(p, ip);
@@ -357,10 +386,14 @@ k_spectrum(const KParams p) {
               }
             }
           }
-          const int64_t g = (int64_t)slot * p.n_grid + (int64_t)dir * p.n_omega + wk;
+          if (p.acc_lds) {
+            s_acc[k * T] += v[0];
+          } else {
+            const int64_t g = (int64_t)slot * p.n_grid + (int64_t)dir * p.n_omega + wk;
 #pragma unroll
-          for (int c = 0; c < kMaxOut; c++)
-            if (c < p.n_out) dst[c][g] += v[c];
+            for (int c = 0; c < kMaxOut; c++)
+              if (c < p.n_out) dst[c][g] += v[c];
+          }
         }
       };
 
@@ -439,8 +472,8 @@ k_spectrum(const KParams p) {
         }
       };
 
-      // one staged step of the direct sweeps for the omegas k with gate(k)
-      auto direct_step = [&](int j, auto gate) {
+      // one staged step of the direct sweeps
+      auto direct_step = [&](int j) {
         const double tau = s_tau[j];
         const double thr = kSplit ? s_thr[j] : 0.0;
         Amp a[NA];
@@ -448,7 +481,6 @@ k_spectrum(const KParams p) {
         for (int c = 0; c < NA; c++) a[c] = s_amp[c][j];
 #pragma unroll
         for (int k = 0; k < M; k++) {
-          if (!gate(k)) continue;
           // FP64: phase in radians and the reference's exact test
           // |phase_s - phase_{s-1}| < pi.  Split: phase in revolutions
           // (omega/2pi precomputed) and the same inequality in its
@@ -585,8 +617,7 @@ k_spectrum(const KParams p) {
         // that keep some omega at step s; a stable counting sort puts the steps
         // in descending bucket order, so run r of rank rho sweeps exactly the
         // first s_cntb[rho] steps.  Steps of bucket 0 are needed by no run.
-        const bool sort_tile = kCompact && (!kRec || rec_sortable) &&
-                               uni((cur.next >= 0 && cur.next < t0 + cnt) ? 0 : 1);
+        const bool sort_tile = kCompact && rec_sortable && uni((cur.next >= 0 && cur.next < t0 + cnt) ? 0 : 1);
         int pos = tid;
         if (sort_tile) {
           const int R = W * M;
@@ -633,31 +664,22 @@ k_spectrum(const KParams p) {
 
         // ---- sweep, split at snapshot records falling inside the tile ----
         if (sort_tile) {
-          // run k of this wave (rank s_rank[k*W + wave]) needs the first ck[k] steps
+          // run k of this wave (rank s_rank[k*W + wave]) needs the first ck[k]
+          // steps; the runs ascend, so ck[0] >= ck[1] >= ... and the steps
+          // [ck[K], ck[K-1]) keep exactly the runs k < K (ck[M] = 0)
           int ck[M];
-          int jmax = 0;
 #pragma unroll
-          for (int k = 0; k < M; k++) {
-            ck[k] = uni(s_cntb[s_rank[k * W + wv]]);
-            jmax = max(jmax, ck[k]);
-          }
-          if (kRec) {
-            // ascending runs: ck[0] >= ck[1] >= ..., so steps [ck[K], ck[K-1])
-            // keep exactly the runs k < K (ck[M] = 0)
-            int j = 0;
-            if (tile_on) {
-              // every valid omega of the block kept at every step: the wave's
-              // valid runs all have ck = cnt, runs past n_omega have ck = 0
-              rec_sweep(IC<M>(), IC<0>(), j, ck[0]);
-            } else {
-              rec_sweep(IC<M>(), IC<2>(), j, ck[M - 1]);
-              if (M >= 2) rec_sweep(IC<(M >= 2 ? M - 1 : 1)>(), IC<2>(), j, ck[M >= 2 ? M - 2 : 0]);
-              if (M >= 3) rec_sweep(IC<(M >= 3 ? M - 2 : 1)>(), IC<2>(), j, ck[M >= 3 ? M - 3 : 0]);
-              if (M >= 4) rec_sweep(IC<(M >= 4 ? M - 3 : 1)>(), IC<2>(), j, ck[0]);
-            }
+          for (int k = 0; k < M; k++) ck[k] = uni(s_cntb[s_rank[k * W + wv]]);
+          int j = 0;
+          if (tile_on) {
+            // every valid omega of the block kept at every step: the wave's
+            // valid runs all have ck = cnt, runs past n_omega have ck = 0
+            rec_sweep(IC<M>(), IC<0>(), j, ck[0]);
           } else {
-#pragma unroll 2
-            for (int j = 0; j < jmax; j++) direct_step(j, [&](int k) { return j < ck[k]; });
+            rec_sweep(IC<M>(), IC<2>(), j, ck[M - 1]);
+            if (M >= 2) rec_sweep(IC<(M >= 2 ? M - 1 : 1)>(), IC<2>(), j, ck[M >= 2 ? M - 2 : 0]);
+            if (M >= 3) rec_sweep(IC<(M >= 3 ? M - 2 : 1)>(), IC<2>(), j, ck[M >= 3 ? M - 3 : 0]);
+            if (M >= 4) rec_sweep(IC<(M >= 4 ? M - 3 : 1)>(), IC<2>(), j, ck[0]);
           }
           kahan_fold();
         } else {
@@ -674,7 +696,7 @@ k_spectrum(const KParams p) {
                 rec_sweep(IC<M>(), IC<1>(), j, jend);
             } else {
 #pragma unroll 2
-              for (; j < jend; j++) direct_step(j, [](int) { return true; });
+              for (; j < jend; j++) direct_step(j);
             }
             kahan_fold();
             if (rec_here) {
@@ -692,6 +714,13 @@ k_spectrum(const KParams p) {
         const int64_t last = cur.next;
         cur.slot++;
         cur.advance(pt, p.n_snaps, last);
+      }
+    }
+    if (p.acc_lds) {
+#pragma unroll 1
+      for (int k = 0; k < M; k++) {
+        const int wk = (int)(chunk * T * M + k * T + tid);
+        if (wk < (int)p.n_omega) dst[0][(int64_t)dir * p.n_omega + wk] += s_acc[k * T];
       }
     }
   }
@@ -758,8 +787,11 @@ Plan make_plan(const sr_grid &g, const sr_opts &o, int64_t n_particles, uint32_t
   Plan pl;
   const int64_t n_om = g.n_omega;
   const int64_t n_dir = (int64_t)g.n_theta * g.n_phi;
+  // AUTO: the recurrence sweep for split spectra on linear grids (measured
+  // fastest there, far and near), the direct sweep otherwise
   pl.sweep = SWEEP_DIRECT;
-  if (o.precision == SR_PREC_SPLIT && g.omega_step != 0.0 && o.sweep == SR_SWEEP_RECURRENCE)
+  if (o.precision == SR_PREC_SPLIT && g.omega_step != 0.0 &&
+      (o.sweep == SR_SWEEP_RECURRENCE || o.sweep == SR_SWEEP_AUTO))
     pl.sweep = SWEEP_REC;
   int best_w = 1, best_m = 1;
   double best_score = -1;
@@ -772,9 +804,8 @@ Plan make_plan(const sr_grid &g, const sr_opts &o, int64_t n_particles, uint32_t
       const int64_t chunks = (n_om + run - 1) / run;
       const double eff = (double)n_om / (double)(chunks * run);
       if (eff < 0.75) continue;
-      // rec: larger M amortises the per-step base sin/cos; M=4 keeps the
-      // Kahan accumulators in registers
-      const double mpref = (pl.sweep == SWEEP_REC) ? (m == 4 ? 0.03 : 0.0) : (m == 2 ? 0.02 : 0.0);
+      // M=2 with 4 waves measured fastest for both sweeps
+      const double mpref = (m == 2) ? 0.02 : 0.0;
       const double score = eff + 0.01 * w + mpref;
       if (score > best_score) { best_score = score; best_w = w; best_m = m; }
     }
@@ -805,7 +836,8 @@ Plan make_plan(const sr_grid &g, const sr_opts &o, int64_t n_particles, uint32_t
 
 template <int MODE, int PREC, int SWEEP, int M>
 hipError_t launch_t(const KParams &kp, int64_t grid, int threads, hipStream_t st) {
-  const size_t lds = (PREC == SR_PREC_SPLIT) ? kahan_lds_bytes<MODE, M>(threads) : 0;
+  const size_t lds = ((PREC == SR_PREC_SPLIT) ? kahan_lds_bytes<MODE, M>(threads) : 0) +
+                     (kp.acc_lds ? acc_lds_bytes<M>(threads) : 0);
   hipLaunchKernelGGL((k_spectrum<MODE, PREC, SWEEP, M>), dim3((unsigned)grid), dim3(threads), lds, st,
                      kp);
   return hipGetLastError();
@@ -915,6 +947,7 @@ int run(KParams &kp, const Plan &pl, const sr_grid &g, const sr_opts &o, void *w
   kp.n_chunks = (uint32_t)((g.n_omega + run - 1) / run);
   kp.slices = pl.slices;
   kp.n_tiles = pl.blocks;
+  kp.acc_lds = (kp.n_out == 1 && kp.n_snaps == 1) ? 1 : 0;
   const size_t need = workspace_for(pl, kp.n_out, kp.n_grid, kp.n_snaps);
   hipError_t e;
   if (need) {
