@@ -129,8 +129,16 @@ struct KParams {
 // its own column, so no barrier guards them.
 template <int MODE, int M>
 constexpr size_t kahan_lds_bytes(int T) {
-  return (size_t)2 * (MODE == SR_MODE_FAR ? 6 : 12) * M * T * sizeof(float);
+  return (size_t)2 * (MODE == SR_MODE_FAR ? 4 : 12) * M * T * sizeof(float);
 }
+
+// Accumulator word of amplitude word c: far [A cos | A sin] with NA words
+// each (split: A.e_theta, A.e_phi; FP64: A_x, A_y, A_z); near
+// [v1 cos | v1 sin | v2 cos | v2 sin] with 3 words each.
+template <int MODE, int NA>
+__device__ constexpr int wcos(int c) { return MODE == SR_MODE_FAR ? c : 6 * (c / 3) + (c % 3); }
+template <int MODE, int NA>
+__device__ constexpr int wsin(int c) { return MODE == SR_MODE_FAR ? c + NA : 6 * (c / 3) + (c % 3) + 3; }
 // Dynamic LDS after the Kahan state: the per-thread particle sums of the
 // single output when KParams::acc_lds is set, [M][T] doubles.
 template <int M>
@@ -149,6 +157,9 @@ constexpr size_t acc_lds_bytes(int T) {
 #endif
 #ifndef SR_REC_WAVES_FAR2
 #define SR_REC_WAVES_FAR2 4
+#endif
+#ifndef SR_REC_WAVES_FAR4
+#define SR_REC_WAVES_FAR4 2
 #endif
 #ifndef SR_REC_GROUP
 #define SR_REC_GROUP 4
@@ -172,7 +183,9 @@ template <int MODE, int PREC, int SWEEP, int M>
 struct WavesPerEU {
   static constexpr int value =
       (PREC == SR_PREC_F64) ? SR_F64_WAVES
-      : (SWEEP == SWEEP_REC) ? ((MODE == SR_MODE_FAR && M == 2) ? SR_REC_WAVES_FAR2 : SR_REC_WAVES)
+      : (SWEEP == SWEEP_REC) ? ((MODE == SR_MODE_FAR && M == 2)   ? SR_REC_WAVES_FAR2
+                                : (MODE == SR_MODE_FAR && M == 4) ? SR_REC_WAVES_FAR4
+                                                                  : SR_REC_WAVES)
                              : (M >= 4 ? 2 : SR_DIRECT_WAVES);
 };
 
@@ -185,7 +198,11 @@ k_spectrum(const KParams p) {
   using Amp = typename std::conditional<PREC == SR_PREC_SPLIT, float, double>::type;
   constexpr bool kSplit = (PREC == SR_PREC_SPLIT);
   constexpr bool kRec = (SWEEP == SWEEP_REC);
-  constexpr int NA = (MODE == SR_MODE_FAR) ? 3 : 6;   // amplitude words per step
+  // amplitude words per step.  Far split: A is perpendicular to n
+  // (n.A = c1 (1 - n.bbar) - c2 n.a = 0 with c1 = (n.a) c2^2), so it is kept
+  // as its two components on e_theta, e_phi.  Far FP64: A_x, A_y, A_z as the
+  // reference sums them.  Near: v1 = (b - n)/R and v2 = n/R^2.
+  constexpr int NA = (MODE == SR_MODE_FAR) ? (kSplit ? 2 : 3) : 6;
   constexpr int NC = 2 * NA;                           // real accumulators per omega
   constexpr bool kCompact = kSplit && kRec && (SR_COMPACT != 0);
   constexpr int kMaxW = kMaxT / kWave;                 // waves per block
@@ -354,7 +371,16 @@ k_spectrum(const KParams p) {
           for (int c = 0; c < NC; c++)
             F[c] = kSplit ? ((double)s_ks[(c * M + k) * T] - (double)s_kc[(c * M + k) * T]) : fs[0][c];
           double re[3], im[3];
-          if (MODE == SR_MODE_FAR) {
+          if constexpr (MODE == SR_MODE_FAR && kSplit) {
+            // F = F_theta e_theta + F_phi e_phi
+            if (spheric) {
+              re[0] = 0.0; re[1] = F[0]; re[2] = F[1];
+              im[0] = 0.0; im[1] = F[2]; im[2] = F[3];
+            } else {
+              re[0] = F[0] * thx + F[1] * phx; re[1] = F[0] * thy + F[1] * phy; re[2] = F[0] * thz;
+              im[0] = F[2] * thx + F[3] * phx; im[1] = F[2] * thy + F[3] * phy; im[2] = F[2] * thz;
+            }
+          } else if constexpr (MODE == SR_MODE_FAR) {
 #pragma unroll
             for (int c = 0; c < 3; c++) { re[c] = F[c]; im[c] = F[3 + c]; }
           } else {
@@ -419,9 +445,8 @@ k_spectrum(const KParams p) {
           }
 #pragma unroll
           for (int c = 0; c < NA; c++) {
-            const int w = 6 * (c / 3) + (c % 3);
-            ps[k][w] = __builtin_fmaf(a[c], cm, ps[k][w]);
-            ps[k][w + 3] = __builtin_fmaf(a[c], sm, ps[k][w + 3]);
+            ps[k][wcos<MODE, NA>(c)] = __builtin_fmaf(a[c], cm, ps[k][wcos<MODE, NA>(c)]);
+            ps[k][wsin<MODE, NA>(c)] = __builtin_fmaf(a[c], sm, ps[k][wsin<MODE, NA>(c)]);
           }
           if (k + 1 < K) {
             const float cn2 = __builtin_fmaf(cn, er, -sn * ei);
@@ -499,22 +524,18 @@ k_spectrum(const KParams p) {
             if (kSplit) {
               float sn, cn;
               sincos_rev(ph, &sn, &cn);
-              // word layout shared with the FP64 path: per amplitude vector
-              // v (far: A; near: v1 then v2) [v cos | v sin], 3 words each
 #pragma unroll
               for (int c = 0; c < NA; c++) {
-                const int w = 6 * (c / 3) + (c % 3);
-                ps[k][w] = __builtin_fmaf((float)a[c], cn, ps[k][w]);
-                ps[k][w + 3] = __builtin_fmaf((float)a[c], sn, ps[k][w + 3]);
+                ps[k][wcos<MODE, NA>(c)] = __builtin_fmaf((float)a[c], cn, ps[k][wcos<MODE, NA>(c)]);
+                ps[k][wsin<MODE, NA>(c)] = __builtin_fmaf((float)a[c], sn, ps[k][wsin<MODE, NA>(c)]);
               }
             } else {
               double sn, cn;
               sincos_reduced(ph, &sn, &cn);
 #pragma unroll
               for (int c = 0; c < NA; c++) {
-                const int w = 6 * (c / 3) + (c % 3);
-                fs[k][w] = fma((double)a[c], cn, fs[k][w]);
-                fs[k][w + 3] = fma((double)a[c], sn, fs[k][w + 3]);
+                fs[k][wcos<MODE, NA>(c)] = fma((double)a[c], cn, fs[k][wcos<MODE, NA>(c)]);
+                fs[k][wsin<MODE, NA>(c)] = fma((double)a[c], sn, fs[k][wsin<MODE, NA>(c)]);
               }
             }
           }
@@ -573,16 +594,22 @@ k_spectrum(const KParams p) {
             double Ax = c1 * (gx - mx) - c2 * ax;
             double Ay = c1 * (gy - my) - c2 * ay;
             double Az = c1 * (gz - mz) - c2 * az;
-            if (spheric) {
-              // kernel_farfield.cl:442-445
-              const double r0 = gx * Ax + gy * Ay + gz * Az;
-              const double r1 = thx * Ax + thy * Ay + thz * Az;
-              const double r2 = phx * Ax + phy * Ay + 0.0 * Az;
-              Ax = r0; Ay = r1; Az = r2;
+            if constexpr (kSplit) {
+              // the two transverse components; the record rebuilds x, y, z
+              st_amp[0] = (Amp)(thx * Ax + thy * Ay + thz * Az);
+              st_amp[1] = (Amp)(phx * Ax + phy * Ay);
+            } else {
+              if (spheric) {
+                // kernel_farfield.cl:442-445
+                const double r0 = gx * Ax + gy * Ay + gz * Az;
+                const double r1 = thx * Ax + thy * Ay + thz * Az;
+                const double r2 = phx * Ax + phy * Ay + 0.0 * Az;
+                Ax = r0; Ay = r1; Az = r2;
+              }
+              st_amp[0] = (Amp)Ax;
+              st_amp[1] = (Amp)Ay;
+              st_amp[2] = (Amp)Az;
             }
-            st_amp[0] = (Amp)Ax;
-            st_amp[1] = (Amp)Ay;
-            st_amp[2] = (Amp)Az;
           } else {
             // kernel_nearfield.cl:67-88; omega of c1 is applied at the record
             const double rx = gx - xs, ry = gy - ys, rz = gz - zs;
