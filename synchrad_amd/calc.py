@@ -28,7 +28,7 @@ import numpy as np
 from .utils import Utilities
 from . import engine
 
-try:  # h5 track/spectrum files need h5py, which is optional here
+try:  # h5 track/spectrum files: h5py when present, else the HDF5 C library
     import h5py  # type: ignore
     h5py_installed = True
 except ImportError:  # pragma: no cover - depends on the image
@@ -38,9 +38,12 @@ except ImportError:  # pragma: no cover - depends on the image
 mpi_installed = False
 
 
-def _need_h5py(what: str):
-    if not h5py_installed:
-        raise ImportError(f"{what} needs h5py, which is not installed")
+def _h5_file(path: str, mode: str):
+    """An h5py.File, or the ctypes binding of libhdf5 with the same use."""
+    if h5py_installed:
+        return h5py.File(path, mode)
+    from . import h5io
+    return h5io.File(path, mode)
 
 
 class SynchRad(Utilities):
@@ -85,8 +88,7 @@ class SynchRad(Utilities):
 
         if file_tracks is not None:
             # synchrad/calc.py:186-219
-            _need_h5py("file_tracks")
-            with h5py.File(file_tracks, "r") as f_tracks:
+            with _h5_file(file_tracks, "r") as f_tracks:
                 self.Args["timeStep"] = self.dtype(f_tracks["misc/cdt"][()])
                 if it_range is None:
                     if "it_range" in f_tracks["misc"].keys():
@@ -145,8 +147,7 @@ class SynchRad(Utilities):
 
         if file_spectrum is not None:
             # synchrad/calc.py:274-290
-            _need_h5py("file_spectrum")
-            with h5py.File(file_spectrum, "w") as f_out:
+            with _h5_file(file_spectrum, "w") as f_out:
                 for key in self.Data["radiation"].keys():
                     f_out["radiation/" + key] = self.Data["radiation"][key]
                 for key in self.Args.keys():
@@ -292,11 +293,10 @@ class SynchRad(Utilities):
 
     def _read_args(self, file_spectrum):
         """Reload a spectrum file, synchrad/calc.py:648-666."""
-        _need_h5py("file_spectrum")
         self.device = None
         self.Data = {"radiation": {}}
         self.Args = {}
-        with h5py.File(file_spectrum, "r") as f:
+        with _h5_file(file_spectrum, "r") as f:
             for key in f["radiation"].keys():
                 self.Data["radiation"][key] = f["radiation/" + key][()]
             for key in f["Args"].keys():
