@@ -164,6 +164,10 @@ constexpr size_t acc_lds_bytes(int T) {
 #ifndef SR_REC_GROUP
 #define SR_REC_GROUP 4
 #endif
+// recurrence groups of 4 steps: all base sin/cos first, then the sums
+#ifndef SR_REC_PIPE
+#define SR_REC_PIPE 1
+#endif
 #ifndef SR_DIRECT_WAVES
 #define SR_DIRECT_WAVES 3
 #endif
@@ -429,12 +433,11 @@ k_spectrum(const KParams p) {
       // skipped when the whole block is above its threshold and each omega is
       // masked otherwise; 2: only the K-th omega is masked (sorted tiles, where
       // the runs below the step's last kept run keep all their omegas).
-      auto rec_step = [&](auto Kc, auto Mc, double tau, double thr, float er, float ei, const float *a) {
+      // rec_acc: the accumulations of one step from its base sin/cos
+      auto rec_acc = [&](auto Kc, auto Mc, float sn, float cn, double thr, float er, float ei,
+                         const float *a) {
         constexpr int K = decltype(Kc)::value;
         constexpr int MASK = decltype(Mc)::value;
-        if (MASK == 1 && thr <= w_lo) return;        // every omega of the block filtered out
-        float sn, cn;
-        sincos_rev(om0_rev * tau, &sn, &cn);
 #pragma unroll
         for (int k = 0; k < K; k++) {
           float cm = cn, sm = sn;
@@ -454,6 +457,12 @@ k_spectrum(const KParams p) {
             cn = cn2;
           }
         }
+      };
+      auto rec_step = [&](auto Kc, auto Mc, double tau, double thr, float er, float ei, const float *a) {
+        if (decltype(Mc)::value == 1 && thr <= w_lo) return;   // every omega of the block filtered out
+        float sn, cn;
+        sincos_rev(om0_rev * tau, &sn, &cn);
+        rec_acc(Kc, Mc, sn, cn, thr, er, ei, a);
       };
       // steps [j, jend) of the staged tile, single steps up to a multiple of 4,
       // then groups of 4 fetched with 16-byte LDS reads, then the tail
@@ -476,18 +485,40 @@ k_spectrum(const KParams p) {
 #pragma unroll
           for (int c = 0; c < NA; c++) av[c] = *(const float4 *)&s_amp[c][j];
           float a[NA];
+          if (SR_REC_PIPE && decltype(Mc)::value != 1) {
+            // the four base phases and sin/cos first (independent chains), then
+            // the accumulations
+            float sn[4], cn[4];
+            sincos_rev(om0_rev * t01.x, &sn[0], &cn[0]);
+            sincos_rev(om0_rev * t01.y, &sn[1], &cn[1]);
+            sincos_rev(om0_rev * t23.x, &sn[2], &cn[2]);
+            sincos_rev(om0_rev * t23.y, &sn[3], &cn[3]);
 #pragma unroll
-          for (int c = 0; c < NA; c++) a[c] = av[c].x;
-          rec_step(Kc, Mc, t01.x, h01.x, er.x, ei.x, a);
+            for (int c = 0; c < NA; c++) a[c] = av[c].x;
+            rec_acc(Kc, Mc, sn[0], cn[0], h01.x, er.x, ei.x, a);
 #pragma unroll
-          for (int c = 0; c < NA; c++) a[c] = av[c].y;
-          rec_step(Kc, Mc, t01.y, h01.y, er.y, ei.y, a);
+            for (int c = 0; c < NA; c++) a[c] = av[c].y;
+            rec_acc(Kc, Mc, sn[1], cn[1], h01.y, er.y, ei.y, a);
 #pragma unroll
-          for (int c = 0; c < NA; c++) a[c] = av[c].z;
-          rec_step(Kc, Mc, t23.x, h23.x, er.z, ei.z, a);
+            for (int c = 0; c < NA; c++) a[c] = av[c].z;
+            rec_acc(Kc, Mc, sn[2], cn[2], h23.x, er.z, ei.z, a);
 #pragma unroll
-          for (int c = 0; c < NA; c++) a[c] = av[c].w;
-          rec_step(Kc, Mc, t23.y, h23.y, er.w, ei.w, a);
+            for (int c = 0; c < NA; c++) a[c] = av[c].w;
+            rec_acc(Kc, Mc, sn[3], cn[3], h23.y, er.w, ei.w, a);
+          } else {
+#pragma unroll
+            for (int c = 0; c < NA; c++) a[c] = av[c].x;
+            rec_step(Kc, Mc, t01.x, h01.x, er.x, ei.x, a);
+#pragma unroll
+            for (int c = 0; c < NA; c++) a[c] = av[c].y;
+            rec_step(Kc, Mc, t01.y, h01.y, er.y, ei.y, a);
+#pragma unroll
+            for (int c = 0; c < NA; c++) a[c] = av[c].z;
+            rec_step(Kc, Mc, t23.x, h23.x, er.z, ei.z, a);
+#pragma unroll
+            for (int c = 0; c < NA; c++) a[c] = av[c].w;
+            rec_step(Kc, Mc, t23.y, h23.y, er.w, ei.w, a);
+          }
         }
         for (; j < jend; j++) {
           float a[NA];
