@@ -36,11 +36,56 @@ __device__ __forceinline__ double unid(double v) {
   return __longlong_as_double(uni64(__double_as_longlong(v)));
 }
 
+// FP64 sin and cos of a phase.  The quadrant k = rint(2 ph / pi) is removed
+// with pi/2 split in two (hi = fl(pi/2), lo = pi/2 - hi; the FMA forms
+// ph - k hi exactly before rounding), leaving |r| <= pi/4 (+ rounding) for
+// |k| < 2^26.  sin r and cos r are their Taylor series to r^17 and r^16 (the
+// first omitted terms are below 1e-19 there), evaluated by Horner in FMA; the
+// quadrant then picks and signs them.  Accurate to a few units in the last
+// place, which the FP64 sums' own rounding dominates.
+#ifndef SR_F64_SINCOS
+#define SR_F64_SINCOS 1
+#endif
+constexpr double kTwoOverPi = 0.6366197723675814;
+constexpr double kPiHalfHi = 1.5707963267948966;
+constexpr double kPiHalfLo = 6.123233995736766e-17;
+
 __device__ __forceinline__ void sincos_reduced(double ph, double *s, double *c) {
+#if SR_F64_SINCOS
+  const double k = rint(ph * kTwoOverPi);
+  double r = fma(-k, kPiHalfHi, ph);
+  r = fma(-k, kPiHalfLo, r);
+  const double r2 = r * r;
+  // 1/n! for n = 17, 15, ..., 3 and 16, 14, ..., 2
+  double ps = 2.8114572543455206e-15;
+  ps = fma(ps, r2, -7.647163731819816e-13);
+  ps = fma(ps, r2, 1.6059043836821613e-10);
+  ps = fma(ps, r2, -2.505210838544172e-08);
+  ps = fma(ps, r2, 2.7557319223985893e-06);
+  ps = fma(ps, r2, -0.0001984126984126984);
+  ps = fma(ps, r2, 0.008333333333333333);
+  ps = fma(ps, r2, -0.16666666666666666);
+  const double sr = fma(ps * r2, r, r);
+  double pc = 4.779477332387385e-14;
+  pc = fma(pc, r2, -1.1470745597729725e-11);
+  pc = fma(pc, r2, 2.08767569878681e-09);
+  pc = fma(pc, r2, -2.755731922398589e-07);
+  pc = fma(pc, r2, 2.48015873015873e-05);
+  pc = fma(pc, r2, -0.001388888888888889);
+  pc = fma(pc, r2, 0.041666666666666664);
+  pc = fma(pc, r2, -0.5);
+  const double cr = fma(pc, r2, 1.0);
+  const int q = (int)(long long)k & 3;
+  const double sq = (q & 1) ? cr : sr;
+  const double cq = (q & 1) ? sr : cr;
+  *s = (q & 2) ? -sq : sq;
+  *c = ((q + 1) & 2) ? -cq : cq;
+#else
   const double k = rint(ph * kInv2Pi);
   double r = fma(-k, k2PiHi, ph);
   r = fma(-k, k2PiLo, r);
   sincos(r, s, c);
+#endif
 }
 
 // sin/cos of 2*pi*rv from an FP64 count of revolutions: the fraction is taken
