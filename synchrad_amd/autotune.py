@@ -26,8 +26,8 @@ _cache: Dict[str, dict] = {}
 
 def candidates(mode: str, precision: str, uniform: bool) -> List[engine.Tuning]:
     T = engine.Tuning
-    if engine.PRECISIONS[precision] == 0:  # FP64: one omega per thread
-        return [T(sweep="direct", omega_per_lane=1, waves_per_block=w) for w in (1, 2, 4)]
+    if engine.PRECISIONS[precision] == 0:  # FP64: one or two omegas per thread, direct sweep
+        return [T(sweep="direct", omega_per_lane=m, waves_per_block=w) for m in (1, 2) for w in (1, 2, 4)]
     ms = (1, 2, 4) if mode == "far" else (1, 2)
     out = [T(sweep="direct", omega_per_lane=m, waves_per_block=w) for m in ms for w in (1, 2, 4)]
     if uniform:

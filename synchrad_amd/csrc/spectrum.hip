@@ -174,6 +174,9 @@ constexpr size_t acc_lds_bytes(int T) {
 #ifndef SR_F64_WAVES
 #define SR_F64_WAVES 4
 #endif
+#ifndef SR_F64_WAVES2
+#define SR_F64_WAVES2 4
+#endif
 // Recurrence sweep: the steps of a tile are put in an order in which every
 // 64-omega run of the block finds the steps that keep any of its omegas first,
 // so a wave sweeps only those (a stable counting sort per tile).  The direct
@@ -186,7 +189,7 @@ constexpr size_t acc_lds_bytes(int T) {
 template <int MODE, int PREC, int SWEEP, int M>
 struct WavesPerEU {
   static constexpr int value =
-      (PREC == SR_PREC_F64) ? SR_F64_WAVES
+      (PREC == SR_PREC_F64) ? (M == 2 ? SR_F64_WAVES2 : SR_F64_WAVES)
       : (SWEEP == SWEEP_REC) ? ((MODE == SR_MODE_FAR && M == 2)   ? SR_REC_WAVES_FAR2
                                 : (MODE == SR_MODE_FAR && M == 4) ? SR_REC_WAVES_FAR4
                                                                   : SR_REC_WAVES)
@@ -198,7 +201,7 @@ __global__ void __launch_bounds__(kMaxT)
 __attribute__((amdgpu_waves_per_eu(WavesPerEU<MODE, PREC, SWEEP, M>::value)))
 k_spectrum(const KParams p) {
   static_assert(!(PREC == SR_PREC_F64 && SWEEP == SWEEP_REC), "recurrence sweep is split-only");
-  static_assert(PREC == SR_PREC_SPLIT || M == 1, "the FP64 sweep holds one omega per thread");
+  static_assert(PREC == SR_PREC_SPLIT || M <= 2, "the FP64 sweep holds one or two omegas per thread");
   using Amp = typename std::conditional<PREC == SR_PREC_SPLIT, float, double>::type;
   constexpr bool kSplit = (PREC == SR_PREC_SPLIT);
   constexpr bool kRec = (SWEEP == SWEEP_REC);
@@ -364,66 +367,76 @@ k_spectrum(const KParams p) {
       // Records are rare; the split sweeps walk their omegas one at a time
       // (not unrolled, everything re-derived from k) so that the record code
       // does not set the kernel's register budget.
-      auto emit = [&](uint32_t slot) {
-#pragma unroll 1
-        for (int k = 0; k < M; k++) {
-          const int wk = (int)(chunk * T * M + k * T + tid);
-          if (wk >= (int)p.n_omega) continue;
-          const double omk = kSplit ? p.omega[wk] : om[0];
-          double F[NC];
-#pragma unroll
-          for (int c = 0; c < NC; c++)
-            F[c] = kSplit ? ((double)s_ks[(c * M + k) * T] - (double)s_kc[(c * M + k) * T]) : fs[0][c];
-          double re[3], im[3];
-          if constexpr (MODE == SR_MODE_FAR && kSplit) {
-            // F = F_theta e_theta + F_phi e_phi
-            if (spheric) {
-              re[0] = 0.0; re[1] = F[0]; re[2] = F[1];
-              im[0] = 0.0; im[1] = F[2]; im[2] = F[3];
-            } else {
-              re[0] = F[0] * thx + F[1] * phx; re[1] = F[0] * thy + F[1] * phy; re[2] = F[0] * thz;
-              im[0] = F[2] * thx + F[3] * phx; im[1] = F[2] * thy + F[3] * phy; im[2] = F[2] * thz;
-            }
-          } else if constexpr (MODE == SR_MODE_FAR) {
-#pragma unroll
-            for (int c = 0; c < 3; c++) { re[c] = F[c]; im[c] = F[3 + c]; }
+      // one omega's record: F are its sums (FP64), omk its 2 pi omega
+      auto emit_k = [&](uint32_t slot, int k, int wk, const double *F, double omk) {
+        double re[3], im[3];
+        if constexpr (MODE == SR_MODE_FAR && kSplit) {
+          // F = F_theta e_theta + F_phi e_phi
+          if (spheric) {
+            re[0] = 0.0; re[1] = F[0]; re[2] = F[1];
+            im[0] = 0.0; im[1] = F[2]; im[2] = F[3];
           } else {
-            // F = G2 + i omega G1 ; words: G1re 0..2, G1im 3..5, G2re 6..8, G2im 9..11
+            re[0] = F[0] * thx + F[1] * phx; re[1] = F[0] * thy + F[1] * phy; re[2] = F[0] * thz;
+            im[0] = F[2] * thx + F[3] * phx; im[1] = F[2] * thy + F[3] * phy; im[2] = F[2] * thz;
+          }
+        } else if constexpr (MODE == SR_MODE_FAR) {
+#pragma unroll
+          for (int c = 0; c < 3; c++) { re[c] = F[c]; im[c] = F[3 + c]; }
+        } else {
+          // F = G2 + i omega G1 ; words: G1re 0..2, G1im 3..5, G2re 6..8, G2im 9..11
+#pragma unroll
+          for (int c = 0; c < 3; c++) {
+            re[c] = F[6 + c] - omk * F[3 + c];
+            im[c] = F[9 + c] + omk * F[c];
+          }
+        }
+        double v[kMaxOut];
+        switch (p.comp) {
+          case SR_COMP_TOTAL:
+            v[0] = wpdt2 * ((re[0] * re[0] + re[1] * re[1] + re[2] * re[2]) +
+                            (im[0] * im[0] + im[1] * im[1] + im[2] * im[2]));
+            break;
+          case SR_COMP_CARTESIAN:
+          case SR_COMP_SPHERIC:
+#pragma unroll
+            for (int c = 0; c < 3; c++) v[c] = wpdt2 * (re[c] * re[c] + im[c] * im[c]);
+            break;
+          default: {
+            const double ff = (MODE == SR_MODE_FAR && p.comp == SR_COMP_CARTESIAN_COMPLEX)
+                                  ? p.form_factor[wk] : 1.0;
 #pragma unroll
             for (int c = 0; c < 3; c++) {
-              re[c] = F[6 + c] - omk * F[3 + c];
-              im[c] = F[9 + c] + omk * F[c];
+              v[2 * c] = wpdt * (ff * re[c]);
+              v[2 * c + 1] = wpdt * (ff * im[c]);
             }
           }
-          double v[kMaxOut];
-          switch (p.comp) {
-            case SR_COMP_TOTAL:
-              v[0] = wpdt2 * ((re[0] * re[0] + re[1] * re[1] + re[2] * re[2]) +
-                              (im[0] * im[0] + im[1] * im[1] + im[2] * im[2]));
-              break;
-            case SR_COMP_CARTESIAN:
-            case SR_COMP_SPHERIC:
+        }
+        if (p.acc_lds) {
+          s_acc[k * T] += v[0];
+        } else {
+          const int64_t g = (int64_t)slot * p.n_grid + (int64_t)dir * p.n_omega + wk;
 #pragma unroll
-              for (int c = 0; c < 3; c++) v[c] = wpdt2 * (re[c] * re[c] + im[c] * im[c]);
-              break;
-            default: {
-              const double ff = (MODE == SR_MODE_FAR && p.comp == SR_COMP_CARTESIAN_COMPLEX)
-                                    ? p.form_factor[wk] : 1.0;
+          for (int c = 0; c < kMaxOut; c++)
+            if (c < p.n_out) dst[c][g] += v[c];
+        }
+      };
+      auto emit = [&](uint32_t slot) {
+        if constexpr (kSplit) {
+#pragma unroll 1
+          for (int k = 0; k < M; k++) {
+            const int wk = (int)(chunk * T * M + k * T + tid);
+            if (wk >= (int)p.n_omega) continue;
+            double F[NC];
 #pragma unroll
-              for (int c = 0; c < 3; c++) {
-                v[2 * c] = wpdt * (ff * re[c]);
-                v[2 * c + 1] = wpdt * (ff * im[c]);
-              }
-            }
+            for (int c = 0; c < NC; c++)
+              F[c] = (double)s_ks[(c * M + k) * T] - (double)s_kc[(c * M + k) * T];
+            emit_k(slot, k, wk, F, p.omega[wk]);
           }
-          if (p.acc_lds) {
-            s_acc[k * T] += v[0];
-          } else {
-            const int64_t g = (int64_t)slot * p.n_grid + (int64_t)dir * p.n_omega + wk;
+        } else {
+          // FP64: the sums live in registers, so the loop is unrolled
 #pragma unroll
-            for (int c = 0; c < kMaxOut; c++)
-              if (c < p.n_out) dst[c][g] += v[c];
-          }
+          for (int k = 0; k < M; k++)
+            if (valid[k]) emit_k(slot, k, wi[k], fs[k], om[k]);
         }
       };
 
@@ -834,7 +847,7 @@ struct Plan {
 // omegas per thread each variant is built for (the near-field recurrence
 // keeps twice the far field's accumulators, so it stops at 2)
 int allowed_m(int mode, int prec, int sweep, int m) {
-  if (prec == SR_PREC_F64) return m == 1;
+  if (prec == SR_PREC_F64) return m == 1 || m == 2;
   if (sweep == SWEEP_REC) return mode == SR_MODE_FAR ? (m == 2 || m == 4) : m == 2;
   return m == 1 || m == 2 || (mode == SR_MODE_FAR && m == 4);
 }
@@ -904,7 +917,10 @@ hipError_t launch_t(const KParams &kp, int64_t grid, int threads, hipStream_t st
 template <int MODE>
 hipError_t launch_mode(const Plan &pl, int prec, const KParams &kp, int64_t grid, int threads,
                        hipStream_t st) {
-  if (prec == SR_PREC_F64) return launch_t<MODE, SR_PREC_F64, SWEEP_DIRECT, 1>(kp, grid, threads, st);
+  if (prec == SR_PREC_F64) {
+    if (pl.M == 2) return launch_t<MODE, SR_PREC_F64, SWEEP_DIRECT, 2>(kp, grid, threads, st);
+    return launch_t<MODE, SR_PREC_F64, SWEEP_DIRECT, 1>(kp, grid, threads, st);
+  }
   if (pl.sweep == SWEEP_REC) {
     if (MODE == SR_MODE_NEAR || pl.M == 2)
       return launch_t<MODE, SR_PREC_SPLIT, SWEEP_REC, 2>(kp, grid, threads, st);
