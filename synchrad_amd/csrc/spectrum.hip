@@ -168,6 +168,9 @@ constexpr size_t acc_lds_bytes(int T) {
 #ifndef SR_REC_PIPE
 #define SR_REC_PIPE 1
 #endif
+#ifndef SR_WAVE_ROTATE
+#define SR_WAVE_ROTATE 1
+#endif
 #ifndef SR_DIRECT_WAVES
 #define SR_DIRECT_WAVES 3
 #endif
@@ -269,12 +272,17 @@ k_spectrum(const KParams p) {
       gz = p.l_screen;
     }
 
+    // the block's omegas are dealt to its waves in 64-omega runs; the low runs
+    // keep the most steps, so the run -> wave assignment is rotated by the tile
+    // index to spread the heavy waves of co-resident blocks over the SIMDs
+    const int vw = SR_WAVE_ROTATE ? (int)(((int64_t)wv + tile) % W) : wv;
+    const int tw = vw * kWave + lane;   // this thread's omega slot within a run set
     int wi[M];
     bool valid[M];
     double om[M];
 #pragma unroll
     for (int k = 0; k < M; k++) {
-      wi[k] = (int)(chunk * T * M + k * T + tid);
+      wi[k] = (int)(chunk * T * M + k * T + tw);
       valid[k] = wi[k] < (int)p.n_omega;
       om[k] = valid[k] ? p.omega[wi[k]] : 0.0;
     }
@@ -304,7 +312,7 @@ k_spectrum(const KParams p) {
       for (int k = 0; k < M; k++) {
         double v = valid[k] ? fabs(om_rev[k]) : INFINITY;
         for (int o = kWave / 2; o > 0; o >>= 1) v = fmin(v, __shfl_xor(v, o));
-        if (lane == 0) s_wmin[k * W + wv] = v;
+        if (lane == 0) s_wmin[k * W + vw] = v;
       }
       __syncthreads();
       if (tid < W * M) {
@@ -424,7 +432,7 @@ k_spectrum(const KParams p) {
         if constexpr (kSplit) {
 #pragma unroll 1
           for (int k = 0; k < M; k++) {
-            const int wk = (int)(chunk * T * M + k * T + tid);
+            const int wk = (int)(chunk * T * M + k * T + tw);
             if (wk >= (int)p.n_omega) continue;
             double F[NC];
 #pragma unroll
@@ -740,7 +748,7 @@ k_spectrum(const KParams p) {
           // [ck[K], ck[K-1]) keep exactly the runs k < K (ck[M] = 0)
           int ck[M];
 #pragma unroll
-          for (int k = 0; k < M; k++) ck[k] = uni(s_cntb[s_rank[k * W + wv]]);
+          for (int k = 0; k < M; k++) ck[k] = uni(s_cntb[s_rank[k * W + vw]]);
           int j = 0;
           if (tile_on) {
             // every valid omega of the block kept at every step: the wave's
@@ -790,7 +798,7 @@ k_spectrum(const KParams p) {
     if (p.acc_lds) {
 #pragma unroll 1
       for (int k = 0; k < M; k++) {
-        const int wk = (int)(chunk * T * M + k * T + tid);
+        const int wk = (int)(chunk * T * M + k * T + tw);
         if (wk < (int)p.n_omega) dst[0][(int64_t)dir * p.n_omega + wk] += s_acc[k * T];
       }
     }
