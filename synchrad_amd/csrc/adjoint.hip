@@ -14,15 +14,23 @@
 //   dL/du = (dL/db - b (b . dL/db)) / gamma
 // with the |dphase| < pi mask held fixed (it is piecewise constant).
 //
-// Layout: a block owns (particle p, direction group q) and walks the group's
-// directions.  Per direction it recomputes F for every omega (threads = omegas,
-// steps staged in LDS as in the spectrum kernel), turns F into per-omega
-// coefficients H_c F_c kept in LDS, then sweeps the transpose (threads =
-// steps, loop over omegas) and adds the step's dL/dx, dL/dm, dL/da into its
-// group's slab.  No step tile is stored between the passes: both passes
-// recompute them.  Slabs are summed over groups in a fixed order by a second
-// kernel, which also applies the m, a -> b -> u chain: no atomics, and the
-// result does not depend on scheduling.
+// Layout.  The fields F_p(g) of a chunk of particles come from the spectrum
+// tile kernel itself (an internal per-particle output, spectrum.hip), into a
+// workspace of at most ~2 GiB; then a block owns (particle p, direction group
+// q) and walks the group's directions.  Per direction it turns F into per-omega
+// coefficients H_c F_c in LDS, then sweeps the transpose (threads = steps, loop
+// over omegas) and adds the step's dL/dx, dL/dm, dL/da into its group's slab.
+// Slabs are summed over groups in a fixed order by a second kernel, which also
+// applies the m, a -> b -> u chain: no atomics, and the result does not depend
+// on scheduling.
+//
+// Split precision, ascending non-negative linear grid: a step keeps exactly the
+// omegas below its threshold |omega|/2pi < 1/(2 |tau_s - tau_{s-1}|) (the
+// forward's test), a prefix of the table found by binary search; exp(i omega
+// tau) walks the prefix by rotation with exp(i domega tau), restarted from an
+// FP64 phase every 32 omegas.  'total' keeps the coefficients on e_theta,
+// e_phi (F is perpendicular to n).  FP64 keeps the reference's exact per-pair
+// test and an FP64 sin/cos per pair.
 #include <hip/hip_runtime.h>
 
 #include <stdint.h>
@@ -57,7 +65,11 @@ struct AParams {
   int32_t comp;
   uint32_t n_omega, n_theta, n_phi, n_dir;
   const double *omega, *sin_t, *cos_t, *sin_p, *cos_p;
+  double domega_rev;      // constant omega step / 2 pi, or 0
   const double *dS[3];
+  // fields of particles [chunk_base, chunk_base + chunk_n): F[6][chunk_n][n_grid]
+  const double *F;
+  int64_t chunk_base, chunk_n, n_grid;
   // decomposition and workspace
   int32_t n_groups;
   double *slab;    // [n_groups][kSlabWords][n_points]
@@ -66,23 +78,25 @@ struct AParams {
   double *gx, *gy, *gz, *gux, *guy, *guz, *gw;
 };
 
-template <int PREC>
+// TWO: split 'total', coefficients on e_theta (word 0) and e_phi (word 1)
+template <int PREC, bool TWO>
 __global__ void __launch_bounds__(kMaxT) k_adjoint(const AParams p) {
   constexpr bool kSplit = (PREC == SR_PREC_SPLIT);
+  static_assert(kSplit || !TWO, "two-word coefficients are split-only");
   using Acc = typename std::conditional<kSplit, float, double>::type;
-  __shared__ double s_tau[kMaxT];
-  __shared__ double s_amp[3][kMaxT];
   __shared__ double s_cw[kAdjMaxOmega];     // omega (F64) or omega/2pi (split)
-  __shared__ Acc s_cf[6][kAdjMaxOmega];     // H_c Re F_c, H_c Im F_c
+  __shared__ Acc s_cf[6][kAdjMaxOmega];     // H_c Re F_c (0..2), H_c Im F_c (3..5)
   __shared__ double s_red[kMaxT / kWave];
 
   const int T = blockDim.x;
   const int tid = threadIdx.x;
-  const int64_t ip = (int64_t)blockIdx.x % p.n_particles;
-  const int q = (int)((int64_t)blockIdx.x / p.n_particles);
+  const int64_t lp = (int64_t)blockIdx.x % p.chunk_n;
+  const int q = (int)((int64_t)blockIdx.x / p.chunk_n);
+  const int64_t ip = p.chunk_base + lp;
   const double dt = p.dt;
   const double dtInv = 1.0 / dt;
-  const double lim = kSplit ? 0.5 : kPi;   // |dphase| bound in the phase unit used
+  constexpr bool two = TWO;
+  constexpr int NCW = TWO ? 2 : 3;   // coefficient words per omega (re and im each)
 
   const Particle pt = load_particle(p, ip);
   double gw_acc = 0.0;
@@ -97,118 +111,117 @@ __global__ void __launch_bounds__(kMaxT) k_adjoint(const AParams p) {
   const uint32_t d_lo = (uint32_t)((int64_t)p.n_dir * q / p.n_groups);
   const uint32_t d_hi = (uint32_t)((int64_t)p.n_dir * (q + 1) / p.n_groups);
   double *slab = p.slab + (int64_t)q * kSlabWords * p.n_points;
+  const int64_t fstride = p.chunk_n * p.n_grid;
 
   for (uint32_t dir = d_lo; dir < d_hi && n_use > 0; dir++) {
     const uint32_t iPhi = dir / p.n_theta;
     const uint32_t iTh = dir - iPhi * p.n_theta;
     const double st = p.sin_t[iTh], ct = p.cos_t[iTh], sp = p.sin_p[iPhi], cp = p.cos_p[iPhi];
     const double nx = st * cp, ny = st * sp, nz = ct;
+    const double et[3] = {ct * cp, ct * sp, -st};   // e_theta
+    const double ep[3] = {-sp, cp, 0.0};            // e_phi
 
-    // step s of the particle: tau (FP64, the reference's expression) and A
-    auto stage = [&](int64_t s, double &tau, double *A, double *m, double *a, double &c1, double &c2) {
+    // ---- coefficient table from the particle's field and dL/dS ----
+    __syncthreads();   // the previous direction is done with the table
+    for (int wi = tid; wi < (int)p.n_omega; wi += T) {
+      const int64_t gidx = (int64_t)dir * p.n_omega + wi;
+      const int64_t fi = lp * p.n_grid + gidx;
+      double Fr[3], Fi[3], G[3];
+#pragma unroll
+      for (int c = 0; c < 3; c++) {
+        Fr[c] = p.F[(int64_t)(2 * c) * fstride + fi];
+        Fi[c] = p.F[(int64_t)(2 * c + 1) * fstride + fi];
+        G[c] = p.dS[p.comp == SR_COMP_TOTAL ? 0 : c][gidx];
+      }
+#pragma unroll
+      for (int c = 0; c < 3; c++) gw_acc += dt * dt * G[c] * (Fr[c] * Fr[c] + Fi[c] * Fi[c]);
+      if constexpr (TWO) {
+        const double H = 2.0 * wdt2 * G[0];
+        s_cf[0][wi] = (Acc)(H * (Fr[0] * et[0] + Fr[1] * et[1] + Fr[2] * et[2]));
+        s_cf[1][wi] = (Acc)(H * (Fr[0] * ep[0] + Fr[1] * ep[1]));
+        s_cf[3][wi] = (Acc)(H * (Fi[0] * et[0] + Fi[1] * et[1] + Fi[2] * et[2]));
+        s_cf[4][wi] = (Acc)(H * (Fi[0] * ep[0] + Fi[1] * ep[1]));
+      } else {
+#pragma unroll
+        for (int c = 0; c < 3; c++) {
+          const double H = 2.0 * wdt2 * G[c];
+          s_cf[c][wi] = (Acc)(H * Fr[c]);
+          s_cf[3 + c][wi] = (Acc)(H * Fi[c]);
+        }
+      }
+      s_cw[wi] = kSplit ? p.omega[wi] * kInv2Pi : p.omega[wi];
+    }
+    __syncthreads();   // table complete
+    const bool rot = kSplit && p.domega_rev > 0.0 && s_cw[0] >= 0.0;
+
+    // ---- transpose sweep: threads = steps, loop over the omega table ----
+    for (int64_t t0 = 0; t0 < n_use; t0 += T) {
+      const int64_t s = t0 + tid;
+      if (s >= n_use) continue;
       const int64_t gs = pt.off + s;
       const double time = (double)(uint32_t)(pt.its + (uint32_t)s) * dt;
-      tau = time - (p.x[gs] * nx + p.y[gs] * ny + p.z[gs] * nz);
+      const double tau = time - (p.x[gs] * nx + p.y[gs] * ny + p.z[gs] * nz);
+      double tau_prev = 0.0;
+      if (s > 0) {
+        const double tp = (double)(uint32_t)(pt.its + (uint32_t)(s - 1)) * dt;
+        tau_prev = tp - (p.x[gs - 1] * nx + p.y[gs - 1] * ny + p.z[gs - 1] * nz);
+      }
       const double u0x = p.ux[gs], u0y = p.uy[gs], u0z = p.uz[gs];
       const double u1x = p.ux[gs + 1], u1y = p.uy[gs + 1], u1z = p.uz[gs + 1];
       const double g0 = gamma_inv(u0x, u0y, u0z), g1 = gamma_inv(u1x, u1y, u1z);
       const double b0x = u0x * g0, b0y = u0y * g0, b0z = u0z * g0;
       const double b1x = u1x * g1, b1y = u1y * g1, b1z = u1z * g1;
-      a[0] = (b1x - b0x) * dtInv; a[1] = (b1y - b0y) * dtInv; a[2] = (b1z - b0z) * dtInv;
-      m[0] = 0.5 * (b1x + b0x); m[1] = 0.5 * (b1y + b0y); m[2] = 0.5 * (b1z + b0z);
-      c1 = a[0] * nx + a[1] * ny + a[2] * nz;
-      c2 = 1.0 - (m[0] * nx + m[1] * ny + m[2] * nz);
+      const double a[3] = {(b1x - b0x) * dtInv, (b1y - b0y) * dtInv, (b1z - b0z) * dtInv};
+      const double m[3] = {0.5 * (b1x + b0x), 0.5 * (b1y + b0y), 0.5 * (b1z + b0z)};
+      double c1 = a[0] * nx + a[1] * ny + a[2] * nz;
+      double c2 = 1.0 - (m[0] * nx + m[1] * ny + m[2] * nz);
       c2 = 1.0 / c2;
       c1 = c1 * c2 * c2;
-      A[0] = c1 * (nx - m[0]) - c2 * a[0];
-      A[1] = c1 * (ny - m[1]) - c2 * a[1];
-      A[2] = c1 * (nz - m[2]) - c2 * a[2];
-    };
+      const double A[3] = {c1 * (nx - m[0]) - c2 * a[0], c1 * (ny - m[1]) - c2 * a[1],
+                           c1 * (nz - m[2]) - c2 * a[2]};
 
-    // ---- forward: F for every omega, then the backward coefficients ----
-    for (int c0 = 0; c0 < (int)p.n_omega; c0 += T) {
-      const int wi = c0 + tid;
-      const bool valid = wi < (int)p.n_omega;
-      const double om = valid ? p.omega[wi] : 0.0;
-      const double wph = kSplit ? om * kInv2Pi : om;
-      double Fr[3] = {0, 0, 0}, Fi[3] = {0, 0, 0};
-      double prev = 0.0;
-      for (int64_t t0 = 0; t0 < n_use; t0 += T) {
-        const int cnt = uni((int)min((int64_t)T, n_use - t0));
-        __syncthreads();
-        if (tid < cnt) {
-          double tau, A[3], m[3], a[3], c1, c2;
-          stage(t0 + tid, tau, A, m, a, c1, c2);
-          s_tau[tid] = tau;
-          s_amp[0][tid] = A[0]; s_amp[1][tid] = A[1]; s_amp[2][tid] = A[2];
-        }
-        __syncthreads();
-        Acc pr[3] = {0, 0, 0}, pi[3] = {0, 0, 0};
-        for (int j = 0; j < cnt; j++) {
-          const double ph = wph * s_tau[j];
-          const double d = ph - prev;
-          prev = ph;
-          if (valid && fabs(d) < lim) {
-            Acc sn, cs;
-            if (kSplit) {
-              float s32, c32;
-              sincos_rev(ph, &s32, &c32);
-              sn = s32; cs = c32;
-            } else {
-              double s64, c64;
-              sincos_reduced(ph, &s64, &c64);
-              sn = s64; cs = c64;
-            }
+      Acc P[3] = {0, 0, 0}, R[3] = {0, 0, 0};
+      auto acc_pair = [&](Acc sn, Acc cs, int j) {
+        const Acc wr = (Acc)s_cw[j];
 #pragma unroll
-            for (int c = 0; c < 3; c++) {
-              const Acc Ac = (Acc)s_amp[c][j];
-              pr[c] = fma(Ac, cs, pr[c]);
-              pi[c] = fma(Ac, sn, pi[c]);
-            }
+        for (int c = 0; c < NCW; c++) {
+          const Acc ar = s_cf[c][j], ai = s_cf[3 + c][j];
+          P[c] = fma(ar, cs, fma(ai, sn, P[c]));
+          R[c] = fma(wr, fma(ar, sn, -ai * cs), R[c]);
+        }
+      };
+      if (rot) {
+        // the omegas this step keeps: those below its threshold (a prefix)
+        const double thr = 0.5 / fabs(tau - tau_prev);
+        int lo = 0, hi = (int)p.n_omega;
+        while (lo < hi) {
+          const int mid = (lo + hi) >> 1;
+          if (s_cw[mid] < thr) lo = mid + 1; else hi = mid;
+        }
+        const int jmax = lo;
+        float er, ei, sn = 0.f, cn = 1.f;
+        sincos_rev(p.domega_rev * tau, &ei, &er);
+        for (int j = 0; j < jmax; j++) {
+          if ((j & 31) == 0) {
+            sincos_rev(s_cw[j] * tau, &sn, &cn);
+          } else {
+            const float cn2 = __builtin_fmaf(cn, er, -sn * ei);
+            sn = __builtin_fmaf(cn, ei, sn * er);
+            cn = cn2;
           }
+          acc_pair((Acc)sn, (Acc)cn, j);
         }
-#pragma unroll
-        for (int c = 0; c < 3; c++) { Fr[c] += (double)pr[c]; Fi[c] += (double)pi[c]; }
-      }
-      if (valid) {
-        const int64_t gidx = (int64_t)dir * p.n_omega + wi;
-        double G[3];
-        if (p.comp == SR_COMP_TOTAL) {
-          G[0] = G[1] = G[2] = p.dS[0][gidx];
-        } else {
-#pragma unroll
-          for (int c = 0; c < 3; c++) G[c] = p.dS[c][gidx];
-        }
-#pragma unroll
-        for (int c = 0; c < 3; c++) {
-          gw_acc += dt * dt * G[c] * (Fr[c] * Fr[c] + Fi[c] * Fi[c]);
-          const double H = 2.0 * wdt2 * G[c];
-          s_cf[c][wi] = (Acc)(H * Fr[c]);
-          s_cf[3 + c][wi] = (Acc)(H * Fi[c]);
-        }
-        s_cw[wi] = wph;
-      }
-    }
-    __syncthreads();   // coefficient table complete
-
-    // ---- backward: threads = steps, loop over the omega table ----
-    for (int64_t t0 = 0; t0 < n_use; t0 += T) {
-      const int64_t s = t0 + tid;
-      if (s < n_use) {
-        double tau, A[3], m[3], a[3], c1, c2;
-        stage(s, tau, A, m, a, c1, c2);
-        double tau_prev = 0.0;
-        if (s > 0) {
-          const int64_t gp = pt.off + s - 1;
-          const double tp = (double)(uint32_t)(pt.its + (uint32_t)(s - 1)) * dt;
-          tau_prev = tp - (p.x[gp] * nx + p.y[gp] * ny + p.z[gp] * nz);
-        }
-        Acc P[3] = {0, 0, 0}, R[3] = {0, 0, 0};
+      } else {
+        const double thr = kSplit ? 0.5 / fabs(tau - tau_prev) : 0.0;
         for (int j = 0; j < (int)p.n_omega; j++) {
           const double w = s_cw[j];
           const double ph = w * tau;
-          const double php = (s > 0) ? w * tau_prev : 0.0;
-          if (!(fabs(ph - php) < lim)) continue;
+          if (kSplit) {
+            if (!(fabs(w) < thr)) continue;   // the forward's threshold form
+          } else {
+            const double php = (s > 0) ? w * tau_prev : 0.0;
+            if (!(fabs(ph - php) < kPi)) continue;   // kernel_farfield.cl:69-72, exactly
+          }
           Acc sn, cs;
           if (kSplit) {
             float s32, c32;
@@ -219,31 +232,33 @@ __global__ void __launch_bounds__(kMaxT) k_adjoint(const AParams p) {
             sincos_reduced(ph, &s64, &c64);
             sn = s64; cs = c64;
           }
-          const Acc wr = (Acc)w;
-#pragma unroll
-          for (int c = 0; c < 3; c++) {
-            const Acc ar = s_cf[c][j], ai = s_cf[3 + c][j];
-            P[c] = fma(ar, cs, fma(ai, sn, P[c]));
-            R[c] = fma(wr, fma(ar, sn, -ai * cs), R[c]);
-          }
-        }
-        const double rs = kSplit ? 2.0 * kPi : 1.0;   // revolutions -> radians
-        const double gA[3] = {(double)P[0], (double)P[1], (double)P[2]};
-        const double AR = rs * (A[0] * (double)R[0] + A[1] * (double)R[1] + A[2] * (double)R[2]);
-        const double nmg = (nx - m[0]) * gA[0] + (ny - m[1]) * gA[1] + (nz - m[2]) * gA[2];
-        const double ag = a[0] * gA[0] + a[1] * gA[1] + a[2] * gA[2];
-        const double nv[3] = {nx, ny, nz};
-        const int64_t i = pt.off + s;
-#pragma unroll
-        for (int c = 0; c < 3; c++) {
-          slab[(int64_t)c * p.n_points + i] += nv[c] * AR;
-          slab[(int64_t)(3 + c) * p.n_points + i] +=
-              2.0 * c1 * c2 * nmg * nv[c] - c1 * gA[c] - c2 * c2 * ag * nv[c];
-          slab[(int64_t)(6 + c) * p.n_points + i] += c2 * c2 * nmg * nv[c] - c2 * gA[c];
+          acc_pair(sn, cs, j);
         }
       }
+      double gA[3], gR[3];
+      if constexpr (TWO) {
+#pragma unroll
+        for (int c = 0; c < 3; c++) {
+          gA[c] = (double)P[0] * et[c] + (double)P[1] * ep[c];
+          gR[c] = (double)R[0] * et[c] + (double)R[1] * ep[c];
+        }
+      } else {
+#pragma unroll
+        for (int c = 0; c < 3; c++) { gA[c] = (double)P[c]; gR[c] = (double)R[c]; }
+      }
+      const double rs = kSplit ? 2.0 * kPi : 1.0;   // revolutions -> radians
+      const double AR = rs * (A[0] * gR[0] + A[1] * gR[1] + A[2] * gR[2]);
+      const double nmg = (nx - m[0]) * gA[0] + (ny - m[1]) * gA[1] + (nz - m[2]) * gA[2];
+      const double ag = a[0] * gA[0] + a[1] * gA[1] + a[2] * gA[2];
+      const double nv[3] = {nx, ny, nz};
+#pragma unroll
+      for (int c = 0; c < 3; c++) {
+        slab[(int64_t)c * p.n_points + gs] += nv[c] * AR;
+        slab[(int64_t)(3 + c) * p.n_points + gs] +=
+            2.0 * c1 * c2 * nmg * nv[c] - c1 * gA[c] - c2 * c2 * ag * nv[c];
+        slab[(int64_t)(6 + c) * p.n_points + gs] += c2 * c2 * nmg * nv[c] - c2 * gA[c];
+      }
     }
-    __syncthreads();   // the next direction rewrites the table
   }
 
   // weight gradient of this (particle, group): block sum in a fixed order
@@ -309,15 +324,21 @@ __global__ void k_adjoint_clear(double *buf, int64_t n) {
 struct APlan {
   int threads;
   int groups;
+  int64_t chunk;   // particles whose fields share the workspace at once
 };
+
+// fields of one chunk of particles: 6 doubles per (particle, point), ~2 GiB at most
+constexpr double kFieldBudget = 2147483648.0;
 
 APlan adjoint_plan(const sr_tracks &t, const sr_grid &g, const sr_opts &o) {
   APlan pl;
-  int waves = o.waves_per_block > 0 ? o.waves_per_block
-                                    : (int)std::min<int64_t>(4, (g.n_omega + kWave - 1) / kWave);
-  pl.threads = kWave * std::max(1, waves);
+  // threads = steps of the transpose sweep
+  const int waves = o.waves_per_block > 0 ? o.waves_per_block : 4;
+  pl.threads = kWave * waves;
   const int64_t n_dir = (int64_t)g.n_theta * g.n_phi;
-  int64_t groups = o.slices > 0 ? o.slices : std::max<int64_t>(1, (4096 + t.n_particles - 1) / t.n_particles);
+  const int64_t n_grid = n_dir * g.n_omega;
+  pl.chunk = std::max<int64_t>(1, std::min<int64_t>(t.n_particles, (int64_t)(kFieldBudget / (48.0 * n_grid))));
+  int64_t groups = o.slices > 0 ? o.slices : std::max<int64_t>(1, (4096 + pl.chunk - 1) / pl.chunk);
   pl.groups = (int)std::min<int64_t>(std::min<int64_t>(groups, n_dir), 1 << 16);
   return pl;
 }
@@ -350,9 +371,13 @@ int adjoint_check(const sr_tracks *t, const sr_grid *g, const sr_opts *o) {
   return SR_OK;
 }
 
-size_t adjoint_ws(const sr_tracks &t, const APlan &pl) {
-  return ((size_t)pl.groups * kSlabWords * (size_t)t.n_points + (size_t)pl.groups * (size_t)t.n_particles) *
-         sizeof(double);
+size_t adjoint_slab_words(const sr_tracks &t, const APlan &pl) {
+  return (size_t)pl.groups * kSlabWords * (size_t)t.n_points + (size_t)pl.groups * (size_t)t.n_particles;
+}
+
+size_t adjoint_ws(const sr_tracks &t, const sr_grid &g, const APlan &pl) {
+  const size_t n_grid = (size_t)g.n_omega * g.n_theta * g.n_phi;
+  return (adjoint_slab_words(t, pl) + 6 * (size_t)pl.chunk * n_grid) * sizeof(double);
 }
 
 }  // namespace
@@ -366,7 +391,7 @@ int sr_far_adjoint_workspace_bytes(const sr_tracks *tracks, const sr_grid *grid,
   int rc = adjoint_check(tracks, grid, opts);
   if (rc) return rc;
   if (tracks->n_particles == 0) return SR_OK;
-  *bytes = adjoint_ws(*tracks, adjoint_plan(*tracks, *grid, *opts));
+  *bytes = adjoint_ws(*tracks, *grid, adjoint_plan(*tracks, *grid, *opts));
   return SR_OK;
 }
 
@@ -382,7 +407,7 @@ int sr_far_adjoint_batch(const sr_tracks *t, const sr_grid *g, const sr_opts *o,
     if (!dS[c]) return fail(SR_ERR_INVALID, "dS[%d] is NULL", c);
   if (!gx || !gy || !gz || !gux || !guy || !guz || !gw) return fail(SR_ERR_INVALID, "a gradient pointer is NULL");
   const APlan pl = adjoint_plan(*t, *g, *o);
-  const size_t need = adjoint_ws(*t, pl);
+  const size_t need = adjoint_ws(*t, *g, pl);
   if (!workspace || workspace_bytes < need)
     return fail(SR_ERR_WORKSPACE, "workspace of %zu bytes needed, %zu given", need, workspace_bytes);
   AParams p;
@@ -400,19 +425,33 @@ int sr_far_adjoint_batch(const sr_tracks *t, const sr_grid *g, const sr_opts *o,
   p.n_omega = g->n_omega; p.n_theta = g->n_theta; p.n_phi = g->n_phi;
   p.n_dir = g->n_theta * g->n_phi;
   p.omega = g->omega; p.sin_t = g->sin_theta; p.cos_t = g->cos_theta; p.sin_p = g->sin_phi; p.cos_p = g->cos_phi;
+  p.domega_rev = g->omega_step * kInv2Pi;
+  p.n_grid = (int64_t)g->n_omega * g->n_theta * g->n_phi;
   for (int c = 0; c < 3; c++) p.dS[c] = dS[c < n_in ? c : 0];
   p.n_groups = pl.groups;
   p.slab = (double *)workspace;
   p.slab_w = p.slab + (size_t)pl.groups * kSlabWords * (size_t)t->n_points;
   p.gx = gx; p.gy = gy; p.gz = gz; p.gux = gux; p.guy = guy; p.guz = guz; p.gw = gw;
+  double *fields = (double *)workspace + adjoint_slab_words(*t, pl);
+  p.F = fields;
   hipStream_t st = (hipStream_t)stream;
-  const int64_t n_clear = (int64_t)(need / sizeof(double));
+  const int64_t n_clear = (int64_t)adjoint_slab_words(*t, pl);
   hipLaunchKernelGGL(k_adjoint_clear, dim3((unsigned)((n_clear + 255) / 256)), dim3(256), 0, st, p.slab, n_clear);
-  const int64_t blocks = t->n_particles * (int64_t)pl.groups;
-  if (o->precision == SR_PREC_F64)
-    hipLaunchKernelGGL(k_adjoint<SR_PREC_F64>, dim3((unsigned)blocks), dim3(pl.threads), 0, st, p);
-  else
-    hipLaunchKernelGGL(k_adjoint<SR_PREC_SPLIT>, dim3((unsigned)blocks), dim3(pl.threads), 0, st, p);
+  for (int64_t pb = 0; pb < t->n_particles; pb += pl.chunk) {
+    const int64_t pe = std::min<int64_t>(t->n_particles, pb + pl.chunk);
+    const int64_t n_f = 6 * (pe - pb) * p.n_grid;
+    hipLaunchKernelGGL(k_adjoint_clear, dim3((unsigned)((n_f + 255) / 256)), dim3(256), 0, st, fields, n_f);
+    if ((rc = spectrum_fields(*t, *g, o->precision, pb, pe, fields, st))) return rc;
+    p.chunk_base = pb;
+    p.chunk_n = pe - pb;
+    const int64_t blocks = p.chunk_n * (int64_t)pl.groups;
+    if (o->precision == SR_PREC_F64)
+      hipLaunchKernelGGL((k_adjoint<SR_PREC_F64, false>), dim3((unsigned)blocks), dim3(pl.threads), 0, st, p);
+    else if (o->comp == SR_COMP_TOTAL)
+      hipLaunchKernelGGL((k_adjoint<SR_PREC_SPLIT, true>), dim3((unsigned)blocks), dim3(pl.threads), 0, st, p);
+    else
+      hipLaunchKernelGGL((k_adjoint<SR_PREC_SPLIT, false>), dim3((unsigned)blocks), dim3(pl.threads), 0, st, p);
+  }
   hipLaunchKernelGGL(k_adjoint_reduce, dim3((unsigned)((t->n_points + 255) / 256)), dim3(256), 0, st, p);
   hipLaunchKernelGGL(k_adjoint_reduce_w, dim3((unsigned)((t->n_particles + 255) / 256)), dim3(256), 0, st, p);
   const hipError_t e = hipGetLastError();

@@ -7,6 +7,8 @@
 #include <stdint.h>
 #include <math.h>
 
+#include "synchrad_amd.h"
+
 namespace sr {
 
 constexpr int kWave = 64;
@@ -22,6 +24,17 @@ constexpr double k2PiLo = 2.4492935982947064e-16;
 // Records a printf-style message for sr_last_error() and returns `code`.
 int fail(int code, const char *fmt, ...) __attribute__((format(printf, 2, 3)));
 const char *last_error();
+
+// Internal output kind of the tile kernel: the complex far field
+// F_p(g) = sum_s A_s exp(i phi_s) of each particle (no weight, no dt), as
+// x re, x im, y re, y im, z re, z im.
+constexpr int kCompField = 100;
+
+// Adds the fields of particles [p_begin, p_end) (one snapshot record each) into
+// F[6][p_end - p_begin][n_grid] with the tile kernel's own plan for `precision`
+// (spectrum.hip).  F must be cleared by the caller; stream-ordered.
+int spectrum_fields(const sr_tracks &t, const sr_grid &g, int32_t precision, int64_t p_begin, int64_t p_end,
+                    double *F, hipStream_t st);
 
 // ---------------------------------------------------------------------------
 // device helpers

@@ -113,6 +113,9 @@ struct KParams {
   // 1: one snapshot and one output -- each thread sums its points over the
   // block's particles in LDS and adds the sum to the output once
   int32_t acc_lds;
+  // per-particle outputs (the adjoint's fields): particle ip writes at
+  // (ip - part_base) * part_stride past the usual index; 0 otherwise
+  int64_t part_base, part_stride;
   // tiling
   uint32_t n_chunks;
   uint32_t n_dir;
@@ -330,8 +333,9 @@ k_spectrum(const KParams p) {
 #pragma unroll
     for (int c = 0; c < kMaxOut; c++)
       dst[c] = (c < p.n_out)
-                   ? (p.slices > 1 ? p.scratch + ((int64_t)q * p.n_out + c) * p.n_grid * p.n_snaps
-                                   : p.out[c])
+                   ? ((p.slices > 1 && p.part_stride == 0)
+                        ? p.scratch + ((int64_t)q * p.n_out + c) * p.n_grid * p.n_snaps
+                        : p.out[c])
                    : nullptr;
 
     if (p.acc_lds) {
@@ -400,6 +404,13 @@ k_spectrum(const KParams p) {
         }
         double v[kMaxOut];
         switch (p.comp) {
+          case kCompField:
+#pragma unroll
+            for (int c = 0; c < 3; c++) {
+              v[2 * c] = re[c];
+              v[2 * c + 1] = im[c];
+            }
+            break;
           case SR_COMP_TOTAL:
             v[0] = wpdt2 * ((re[0] * re[0] + re[1] * re[1] + re[2] * re[2]) +
                             (im[0] * im[0] + im[1] * im[1] + im[2] * im[2]));
@@ -422,7 +433,8 @@ k_spectrum(const KParams p) {
         if (p.acc_lds) {
           s_acc[k * T] += v[0];
         } else {
-          const int64_t g = (int64_t)slot * p.n_grid + (int64_t)dir * p.n_omega + wk;
+          const int64_t g = (int64_t)slot * p.n_grid + (int64_t)dir * p.n_omega + wk +
+                            (ip - p.part_base) * p.part_stride;
 #pragma unroll
           for (int c = 0; c < kMaxOut; c++)
             if (c < p.n_out) dst[c][g] += v[c];
@@ -1030,7 +1042,8 @@ int run(KParams &kp, const Plan &pl, const sr_grid &g, const sr_opts &o, void *w
   kp.slices = pl.slices;
   kp.n_tiles = pl.blocks;
   kp.acc_lds = (kp.n_out == 1 && kp.n_snaps == 1) ? 1 : 0;
-  const size_t need = workspace_for(pl, kp.n_out, kp.n_grid, kp.n_snaps);
+  // per-particle outputs are disjoint: slices write them in place
+  const size_t need = kp.part_stride ? 0 : workspace_for(pl, kp.n_out, kp.n_grid, kp.n_snaps);
   hipError_t e;
   if (need) {
     if (!ws || ws_bytes < need)
@@ -1112,6 +1125,39 @@ int single_particle(int32_t mode, int32_t comp, double *const *outs, const doubl
 }
 
 }  // namespace
+
+namespace sr {
+
+int spectrum_fields(const sr_tracks &t, const sr_grid &g, int32_t precision, int64_t p_begin, int64_t p_end,
+                    double *F, hipStream_t st) {
+  KParams kp;
+  memset(&kp, 0, sizeof(kp));
+  kp.offsets = t.offsets;
+  kp.x = t.x; kp.y = t.y; kp.z = t.z; kp.ux = t.ux; kp.uy = t.uy; kp.uz = t.uz;
+  kp.weight = t.weight;
+  kp.it_start = t.it_start;
+  kp.it_end = t.it_end;
+  kp.it_snaps = t.it_snaps;
+  kp.n_snaps = 1;
+  kp.snaps_stride = t.snaps_stride;
+  kp.dt = t.dt;
+  sr_opts o;
+  memset(&o, 0, sizeof(o));
+  o.comp = kCompField;
+  o.precision = precision;
+  const int64_t np = p_end - p_begin;
+  const int64_t n_grid = (int64_t)g.n_omega * g.n_theta * g.n_phi;
+  double *outs[kMaxOut];
+  for (int c = 0; c < 6; c++) outs[c] = F + (int64_t)c * np * n_grid;
+  fill_grid(kp, g, o, 6, outs);
+  const Plan pl = make_plan(g, o, np, 1);
+  for (int q = 0; q <= pl.slices; q++) kp.slice_lo[q] = p_begin + np * q / pl.slices;
+  kp.part_base = p_begin;
+  kp.part_stride = n_grid;
+  return run(kp, pl, g, o, nullptr, 0, st);
+}
+
+}  // namespace sr
 
 // ---------------------------------------------------------------------------
 // C ABI
