@@ -72,6 +72,7 @@ struct AParams {
   int64_t chunk_base, chunk_n, n_grid;
   // decomposition and workspace
   int32_t n_groups;
+  int32_t dir_batch;   // directions whose coefficient tables share the block's LDS
   double *slab;    // [n_groups][kSlabWords][n_points]
   double *slab_w;  // [n_groups][n_particles]
   // outputs (+=)
@@ -84,9 +85,12 @@ __global__ void __launch_bounds__(kMaxT) k_adjoint(const AParams p) {
   constexpr bool kSplit = (PREC == SR_PREC_SPLIT);
   static_assert(kSplit || !TWO, "two-word coefficients are split-only");
   using Acc = typename std::conditional<kSplit, float, double>::type;
+  constexpr int NCW = TWO ? 2 : 3;   // coefficient words per omega: re 0..NCW-1, im NCW..2NCW-1
+  constexpr int NW = 2 * NCW;
   __shared__ double s_cw[kAdjMaxOmega];     // omega (F64) or omega/2pi (split)
-  __shared__ Acc s_cf[6][kAdjMaxOmega];     // H_c Re F_c (0..2), H_c Im F_c (3..5)
   __shared__ double s_red[kMaxT / kWave];
+  extern __shared__ __align__(16) char s_dyn_raw[];
+  Acc *const s_cf = (Acc *)s_dyn_raw;       // [dir_batch][NW][n_omega]: H_c F_c
 
   const int T = blockDim.x;
   const int tid = threadIdx.x;
@@ -95,8 +99,7 @@ __global__ void __launch_bounds__(kMaxT) k_adjoint(const AParams p) {
   const int64_t ip = p.chunk_base + lp;
   const double dt = p.dt;
   const double dtInv = 1.0 / dt;
-  constexpr bool two = TWO;
-  constexpr int NCW = TWO ? 2 : 3;   // coefficient words per omega (re and im each)
+  const int n_om = (int)p.n_omega;
 
   const Particle pt = load_particle(p, ip);
   double gw_acc = 0.0;
@@ -113,17 +116,30 @@ __global__ void __launch_bounds__(kMaxT) k_adjoint(const AParams p) {
   double *slab = p.slab + (int64_t)q * kSlabWords * p.n_points;
   const int64_t fstride = p.chunk_n * p.n_grid;
 
-  for (uint32_t dir = d_lo; dir < d_hi && n_use > 0; dir++) {
+  // geometry of direction dir: n, e_theta, e_phi
+  auto geom = [&](uint32_t dir, double *n, double *et, double *ep) {
     const uint32_t iPhi = dir / p.n_theta;
     const uint32_t iTh = dir - iPhi * p.n_theta;
     const double st = p.sin_t[iTh], ct = p.cos_t[iTh], sp = p.sin_p[iPhi], cp = p.cos_p[iPhi];
-    const double nx = st * cp, ny = st * sp, nz = ct;
-    const double et[3] = {ct * cp, ct * sp, -st};   // e_theta
-    const double ep[3] = {-sp, cp, 0.0};            // e_phi
+    n[0] = st * cp; n[1] = st * sp; n[2] = ct;
+    et[0] = ct * cp; et[1] = ct * sp; et[2] = -st;
+    ep[0] = -sp; ep[1] = cp; ep[2] = 0.0;
+  };
 
-    // ---- coefficient table from the particle's field and dL/dS ----
-    __syncthreads();   // the previous direction is done with the table
-    for (int wi = tid; wi < (int)p.n_omega; wi += T) {
+  // directions are taken in batches whose coefficient tables fit the block's
+  // LDS, so that each step's nine gradient words are summed over a batch in
+  // registers and written once per batch
+  for (uint32_t d0 = d_lo; d0 < d_hi && n_use > 0; d0 += (uint32_t)p.dir_batch) {
+    const int nb = (int)min((uint32_t)p.dir_batch, d_hi - d0);
+
+    // ---- coefficient tables from the particle's fields and dL/dS ----
+    __syncthreads();   // the previous batch is done with the tables
+    for (int idx = tid; idx < nb * n_om; idx += T) {
+      const int d = idx / n_om;
+      const int wi = idx - d * n_om;
+      const uint32_t dir = d0 + (uint32_t)d;
+      double nv[3], et[3], ep[3];
+      geom(dir, nv, et, ep);
       const int64_t gidx = (int64_t)dir * p.n_omega + wi;
       const int64_t fi = lp * p.n_grid + gidx;
       double Fr[3], Fi[3], G[3];
@@ -135,37 +151,39 @@ __global__ void __launch_bounds__(kMaxT) k_adjoint(const AParams p) {
       }
 #pragma unroll
       for (int c = 0; c < 3; c++) gw_acc += dt * dt * G[c] * (Fr[c] * Fr[c] + Fi[c] * Fi[c]);
+      Acc *cf = s_cf + (size_t)d * NW * n_om;
       if constexpr (TWO) {
         const double H = 2.0 * wdt2 * G[0];
-        s_cf[0][wi] = (Acc)(H * (Fr[0] * et[0] + Fr[1] * et[1] + Fr[2] * et[2]));
-        s_cf[1][wi] = (Acc)(H * (Fr[0] * ep[0] + Fr[1] * ep[1]));
-        s_cf[3][wi] = (Acc)(H * (Fi[0] * et[0] + Fi[1] * et[1] + Fi[2] * et[2]));
-        s_cf[4][wi] = (Acc)(H * (Fi[0] * ep[0] + Fi[1] * ep[1]));
+        cf[0 * n_om + wi] = (Acc)(H * (Fr[0] * et[0] + Fr[1] * et[1] + Fr[2] * et[2]));
+        cf[1 * n_om + wi] = (Acc)(H * (Fr[0] * ep[0] + Fr[1] * ep[1]));
+        cf[2 * n_om + wi] = (Acc)(H * (Fi[0] * et[0] + Fi[1] * et[1] + Fi[2] * et[2]));
+        cf[3 * n_om + wi] = (Acc)(H * (Fi[0] * ep[0] + Fi[1] * ep[1]));
       } else {
 #pragma unroll
         for (int c = 0; c < 3; c++) {
           const double H = 2.0 * wdt2 * G[c];
-          s_cf[c][wi] = (Acc)(H * Fr[c]);
-          s_cf[3 + c][wi] = (Acc)(H * Fi[c]);
+          cf[c * n_om + wi] = (Acc)(H * Fr[c]);
+          cf[(3 + c) * n_om + wi] = (Acc)(H * Fi[c]);
         }
       }
-      s_cw[wi] = kSplit ? p.omega[wi] * kInv2Pi : p.omega[wi];
+      if (d == 0) s_cw[wi] = kSplit ? p.omega[wi] * kInv2Pi : p.omega[wi];
     }
-    __syncthreads();   // table complete
+    __syncthreads();   // tables complete
     const bool rot = kSplit && p.domega_rev > 0.0 && s_cw[0] >= 0.0;
 
-    // ---- transpose sweep: threads = steps, loop over the omega table ----
+    // ---- transpose sweep: threads = steps, loop over directions and omegas ----
     for (int64_t t0 = 0; t0 < n_use; t0 += T) {
       const int64_t s = t0 + tid;
       if (s >= n_use) continue;
       const int64_t gs = pt.off + s;
       const double time = (double)(uint32_t)(pt.its + (uint32_t)s) * dt;
-      const double tau = time - (p.x[gs] * nx + p.y[gs] * ny + p.z[gs] * nz);
-      double tau_prev = 0.0;
+      const double xs = p.x[gs], ys = p.y[gs], zs = p.z[gs];
+      double tp = 0.0, xp = 0.0, yp = 0.0, zp = 0.0;
       if (s > 0) {
-        const double tp = (double)(uint32_t)(pt.its + (uint32_t)(s - 1)) * dt;
-        tau_prev = tp - (p.x[gs - 1] * nx + p.y[gs - 1] * ny + p.z[gs - 1] * nz);
+        tp = (double)(uint32_t)(pt.its + (uint32_t)(s - 1)) * dt;
+        xp = p.x[gs - 1]; yp = p.y[gs - 1]; zp = p.z[gs - 1];
       }
+      // direction-independent part of the step (kernel_farfield.cl:74-83)
       const double u0x = p.ux[gs], u0y = p.uy[gs], u0z = p.uz[gs];
       const double u1x = p.ux[gs + 1], u1y = p.uy[gs + 1], u1z = p.uz[gs + 1];
       const double g0 = gamma_inv(u0x, u0y, u0z), g1 = gamma_inv(u1x, u1y, u1z);
@@ -173,91 +191,104 @@ __global__ void __launch_bounds__(kMaxT) k_adjoint(const AParams p) {
       const double b1x = u1x * g1, b1y = u1y * g1, b1z = u1z * g1;
       const double a[3] = {(b1x - b0x) * dtInv, (b1y - b0y) * dtInv, (b1z - b0z) * dtInv};
       const double m[3] = {0.5 * (b1x + b0x), 0.5 * (b1y + b0y), 0.5 * (b1z + b0z)};
-      double c1 = a[0] * nx + a[1] * ny + a[2] * nz;
-      double c2 = 1.0 - (m[0] * nx + m[1] * ny + m[2] * nz);
-      c2 = 1.0 / c2;
-      c1 = c1 * c2 * c2;
-      const double A[3] = {c1 * (nx - m[0]) - c2 * a[0], c1 * (ny - m[1]) - c2 * a[1],
-                           c1 * (nz - m[2]) - c2 * a[2]};
-
-      Acc P[3] = {0, 0, 0}, R[3] = {0, 0, 0};
-      auto acc_pair = [&](Acc sn, Acc cs, int j) {
-        const Acc wr = (Acc)s_cw[j];
+      double acc9[kSlabWords];
 #pragma unroll
-        for (int c = 0; c < NCW; c++) {
-          const Acc ar = s_cf[c][j], ai = s_cf[3 + c][j];
-          P[c] = fma(ar, cs, fma(ai, sn, P[c]));
-          R[c] = fma(wr, fma(ar, sn, -ai * cs), R[c]);
-        }
-      };
-      if (rot) {
-        // the omegas this step keeps: those below its threshold (a prefix)
-        const double thr = 0.5 / fabs(tau - tau_prev);
-        int lo = 0, hi = (int)p.n_omega;
-        while (lo < hi) {
-          const int mid = (lo + hi) >> 1;
-          if (s_cw[mid] < thr) lo = mid + 1; else hi = mid;
-        }
-        const int jmax = lo;
-        float er, ei, sn = 0.f, cn = 1.f;
-        sincos_rev(p.domega_rev * tau, &ei, &er);
-        for (int j = 0; j < jmax; j++) {
-          if ((j & 31) == 0) {
-            sincos_rev(s_cw[j] * tau, &sn, &cn);
-          } else {
-            const float cn2 = __builtin_fmaf(cn, er, -sn * ei);
-            sn = __builtin_fmaf(cn, ei, sn * er);
-            cn = cn2;
+      for (int k = 0; k < kSlabWords; k++) acc9[k] = 0.0;
+
+      for (int d = 0; d < nb; d++) {
+        double nv[3], et[3], ep[3];
+        geom(d0 + (uint32_t)d, nv, et, ep);
+        const Acc *cf = s_cf + (size_t)d * NW * n_om;
+        const double tau = time - (xs * nv[0] + ys * nv[1] + zs * nv[2]);
+        const double tau_prev = (s > 0) ? tp - (xp * nv[0] + yp * nv[1] + zp * nv[2]) : 0.0;
+        double c1 = a[0] * nv[0] + a[1] * nv[1] + a[2] * nv[2];
+        double c2 = 1.0 - (m[0] * nv[0] + m[1] * nv[1] + m[2] * nv[2]);
+        c2 = 1.0 / c2;
+        c1 = c1 * c2 * c2;
+        const double A[3] = {c1 * (nv[0] - m[0]) - c2 * a[0], c1 * (nv[1] - m[1]) - c2 * a[1],
+                             c1 * (nv[2] - m[2]) - c2 * a[2]};
+
+        Acc P[NCW], R[NCW];
+#pragma unroll
+        for (int c = 0; c < NCW; c++) { P[c] = 0; R[c] = 0; }
+        auto acc_pair = [&](Acc sn, Acc cs, int j) {
+          const Acc wr = (Acc)s_cw[j];
+#pragma unroll
+          for (int c = 0; c < NCW; c++) {
+            const Acc ar = cf[c * n_om + j], ai = cf[(NCW + c) * n_om + j];
+            P[c] = fma(ar, cs, fma(ai, sn, P[c]));
+            R[c] = fma(wr, fma(ar, sn, -ai * cs), R[c]);
           }
-          acc_pair((Acc)sn, (Acc)cn, j);
-        }
-      } else {
-        const double thr = kSplit ? 0.5 / fabs(tau - tau_prev) : 0.0;
-        for (int j = 0; j < (int)p.n_omega; j++) {
-          const double w = s_cw[j];
-          const double ph = w * tau;
-          if (kSplit) {
-            if (!(fabs(w) < thr)) continue;   // the forward's threshold form
-          } else {
-            const double php = (s > 0) ? w * tau_prev : 0.0;
-            if (!(fabs(ph - php) < kPi)) continue;   // kernel_farfield.cl:69-72, exactly
+        };
+        if (rot) {
+          // the omegas this step keeps: those below its threshold (a prefix)
+          const double thr = 0.5 / fabs(tau - tau_prev);
+          int lo = 0, hi = n_om;
+          while (lo < hi) {
+            const int mid = (lo + hi) >> 1;
+            if (s_cw[mid] < thr) lo = mid + 1; else hi = mid;
           }
-          Acc sn, cs;
-          if (kSplit) {
-            float s32, c32;
-            sincos_rev(ph, &s32, &c32);
-            sn = s32; cs = c32;
-          } else {
-            double s64, c64;
-            sincos_reduced(ph, &s64, &c64);
-            sn = s64; cs = c64;
+          const int jmax = lo;
+          float er, ei, sn = 0.f, cn = 1.f;
+          sincos_rev(p.domega_rev * tau, &ei, &er);
+          for (int j = 0; j < jmax; j++) {
+            if ((j & 31) == 0) {
+              sincos_rev(s_cw[j] * tau, &sn, &cn);
+            } else {
+              const float cn2 = __builtin_fmaf(cn, er, -sn * ei);
+              sn = __builtin_fmaf(cn, ei, sn * er);
+              cn = cn2;
+            }
+            acc_pair((Acc)sn, (Acc)cn, j);
           }
-          acc_pair(sn, cs, j);
+        } else {
+          const double thr = kSplit ? 0.5 / fabs(tau - tau_prev) : 0.0;
+          for (int j = 0; j < n_om; j++) {
+            const double w = s_cw[j];
+            const double ph = w * tau;
+            if (kSplit) {
+              if (!(fabs(w) < thr)) continue;   // the forward's threshold form
+            } else {
+              const double php = (s > 0) ? w * tau_prev : 0.0;
+              if (!(fabs(ph - php) < kPi)) continue;   // kernel_farfield.cl:69-72, exactly
+            }
+            Acc sn, cs;
+            if (kSplit) {
+              float s32, c32;
+              sincos_rev(ph, &s32, &c32);
+              sn = s32; cs = c32;
+            } else {
+              double s64, c64;
+              sincos_reduced(ph, &s64, &c64);
+              sn = s64; cs = c64;
+            }
+            acc_pair(sn, cs, j);
+          }
         }
-      }
-      double gA[3], gR[3];
-      if constexpr (TWO) {
+        double gA[3], gR[3];
+        if constexpr (TWO) {
+#pragma unroll
+          for (int c = 0; c < 3; c++) {
+            gA[c] = (double)P[0] * et[c] + (double)P[1] * ep[c];
+            gR[c] = (double)R[0] * et[c] + (double)R[1] * ep[c];
+          }
+        } else {
+#pragma unroll
+          for (int c = 0; c < 3; c++) { gA[c] = (double)P[c]; gR[c] = (double)R[c]; }
+        }
+        const double rs = kSplit ? 2.0 * kPi : 1.0;   // revolutions -> radians
+        const double AR = rs * (A[0] * gR[0] + A[1] * gR[1] + A[2] * gR[2]);
+        const double nmg = (nv[0] - m[0]) * gA[0] + (nv[1] - m[1]) * gA[1] + (nv[2] - m[2]) * gA[2];
+        const double ag = a[0] * gA[0] + a[1] * gA[1] + a[2] * gA[2];
 #pragma unroll
         for (int c = 0; c < 3; c++) {
-          gA[c] = (double)P[0] * et[c] + (double)P[1] * ep[c];
-          gR[c] = (double)R[0] * et[c] + (double)R[1] * ep[c];
+          acc9[c] += nv[c] * AR;
+          acc9[3 + c] += 2.0 * c1 * c2 * nmg * nv[c] - c1 * gA[c] - c2 * c2 * ag * nv[c];
+          acc9[6 + c] += c2 * c2 * nmg * nv[c] - c2 * gA[c];
         }
-      } else {
-#pragma unroll
-        for (int c = 0; c < 3; c++) { gA[c] = (double)P[c]; gR[c] = (double)R[c]; }
       }
-      const double rs = kSplit ? 2.0 * kPi : 1.0;   // revolutions -> radians
-      const double AR = rs * (A[0] * gR[0] + A[1] * gR[1] + A[2] * gR[2]);
-      const double nmg = (nx - m[0]) * gA[0] + (ny - m[1]) * gA[1] + (nz - m[2]) * gA[2];
-      const double ag = a[0] * gA[0] + a[1] * gA[1] + a[2] * gA[2];
-      const double nv[3] = {nx, ny, nz};
 #pragma unroll
-      for (int c = 0; c < 3; c++) {
-        slab[(int64_t)c * p.n_points + gs] += nv[c] * AR;
-        slab[(int64_t)(3 + c) * p.n_points + gs] +=
-            2.0 * c1 * c2 * nmg * nv[c] - c1 * gA[c] - c2 * c2 * ag * nv[c];
-        slab[(int64_t)(6 + c) * p.n_points + gs] += c2 * c2 * nmg * nv[c] - c2 * gA[c];
-      }
+      for (int k = 0; k < kSlabWords; k++) slab[(int64_t)k * p.n_points + gs] += acc9[k];
     }
   }
 
@@ -325,7 +356,12 @@ struct APlan {
   int threads;
   int groups;
   int64_t chunk;   // particles whose fields share the workspace at once
+  int dir_batch;   // directions per LDS batch
+  size_t lds;      // dynamic LDS of the sweep kernel
 };
+
+// LDS for the coefficient tables of one direction batch
+constexpr size_t kAdjTableBudget = 32768;
 
 // fields of one chunk of particles: 6 doubles per (particle, point), ~2 GiB at most
 constexpr double kFieldBudget = 2147483648.0;
@@ -340,6 +376,13 @@ APlan adjoint_plan(const sr_tracks &t, const sr_grid &g, const sr_opts &o) {
   pl.chunk = std::max<int64_t>(1, std::min<int64_t>(t.n_particles, (int64_t)(kFieldBudget / (48.0 * n_grid))));
   int64_t groups = o.slices > 0 ? o.slices : std::max<int64_t>(1, (4096 + pl.chunk - 1) / pl.chunk);
   pl.groups = (int)std::min<int64_t>(std::min<int64_t>(groups, n_dir), 1 << 16);
+  const bool split = o.precision == SR_PREC_SPLIT;
+  const size_t per_dir = (size_t)((split && o.comp == SR_COMP_TOTAL) ? 4 : 6) * g.n_omega *
+                         (split ? sizeof(float) : sizeof(double));
+  const int64_t dirs_per_group = (n_dir + pl.groups - 1) / pl.groups;
+  pl.dir_batch = (int)std::max<int64_t>(1, std::min<int64_t>({(int64_t)(kAdjTableBudget / per_dir), 16,
+                                                              dirs_per_group}));
+  pl.lds = per_dir * pl.dir_batch;
   return pl;
 }
 
@@ -429,6 +472,7 @@ int sr_far_adjoint_batch(const sr_tracks *t, const sr_grid *g, const sr_opts *o,
   p.n_grid = (int64_t)g->n_omega * g->n_theta * g->n_phi;
   for (int c = 0; c < 3; c++) p.dS[c] = dS[c < n_in ? c : 0];
   p.n_groups = pl.groups;
+  p.dir_batch = pl.dir_batch;
   p.slab = (double *)workspace;
   p.slab_w = p.slab + (size_t)pl.groups * kSlabWords * (size_t)t->n_points;
   p.gx = gx; p.gy = gy; p.gz = gz; p.gux = gux; p.guy = guy; p.guz = guz; p.gw = gw;
@@ -446,11 +490,11 @@ int sr_far_adjoint_batch(const sr_tracks *t, const sr_grid *g, const sr_opts *o,
     p.chunk_n = pe - pb;
     const int64_t blocks = p.chunk_n * (int64_t)pl.groups;
     if (o->precision == SR_PREC_F64)
-      hipLaunchKernelGGL((k_adjoint<SR_PREC_F64, false>), dim3((unsigned)blocks), dim3(pl.threads), 0, st, p);
+      hipLaunchKernelGGL((k_adjoint<SR_PREC_F64, false>), dim3((unsigned)blocks), dim3(pl.threads), pl.lds, st, p);
     else if (o->comp == SR_COMP_TOTAL)
-      hipLaunchKernelGGL((k_adjoint<SR_PREC_SPLIT, true>), dim3((unsigned)blocks), dim3(pl.threads), 0, st, p);
+      hipLaunchKernelGGL((k_adjoint<SR_PREC_SPLIT, true>), dim3((unsigned)blocks), dim3(pl.threads), pl.lds, st, p);
     else
-      hipLaunchKernelGGL((k_adjoint<SR_PREC_SPLIT, false>), dim3((unsigned)blocks), dim3(pl.threads), 0, st, p);
+      hipLaunchKernelGGL((k_adjoint<SR_PREC_SPLIT, false>), dim3((unsigned)blocks), dim3(pl.threads), pl.lds, st, p);
   }
   hipLaunchKernelGGL(k_adjoint_reduce, dim3((unsigned)((t->n_points + 255) / 256)), dim3(256), 0, st, p);
   hipLaunchKernelGGL(k_adjoint_reduce_w, dim3((unsigned)((t->n_particles + 255) / 256)), dim3(256), 0, st, p);
