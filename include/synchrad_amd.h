@@ -135,9 +135,12 @@ const char *sr_last_error(void);
 int sr_num_outputs(int32_t mode, int32_t comp);
 
 /*
- * Workspace bytes sr_spectrum_batch needs for this request: particle slices
- * of one tile are accumulated in slices*n_out*nSnaps*n_grid doubles and then
- * summed in slice order; 0 when the plan uses no slices.
+ * Workspace bytes sr_spectrum_batch needs for this request: the
+ * direction-independent data of every sample (far: a and the interval's mean
+ * velocity, 6 doubles; near: the velocity, 3 doubles), computed once per call
+ * instead of once per spectral point, plus, when the plan uses particle
+ * slices, slices*n_out*nSnaps*n_grid doubles of partial spectra summed in
+ * slice order.  0 for an empty batch.
  */
 int sr_workspace_bytes(const sr_tracks *tracks, const sr_grid *grid, const sr_opts *opts,
                        size_t *bytes);
@@ -161,7 +164,10 @@ int sr_spectrum_batch(const sr_tracks *tracks, const sr_grid *grid, const sr_opt
  * (device [n_particles]).  The |dphase| < pi mask of the forward is held
  * fixed.  SR_PREC_F64 evaluates everything in FP64; SR_PREC_SPLIT forms
  * phases in FP64 and sin/cos and the omega sums in FP32.  opts.slices sets the
- * number of direction groups (0 = chosen); opts.waves_per_block the block size.
+ * number of direction groups (0 = chosen); opts.waves_per_block the block size
+ * of the transpose sweep (threads = steps).  The workspace holds the group
+ * slabs, the particles' complex fields for a chunk of particles (at most
+ * ~2 GiB; larger batches are processed chunk by chunk) and the step data.
  */
 int sr_far_adjoint_workspace_bytes(const sr_tracks *tracks, const sr_grid *grid, const sr_opts *opts,
                                    size_t *bytes);

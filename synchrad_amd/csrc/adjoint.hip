@@ -69,6 +69,8 @@ struct AParams {
   const double *dS[3];
   // fields of particles [chunk_base, chunk_base + chunk_n): F[6][chunk_n][n_grid]
   const double *F;
+  // step data [a_x, a_y, a_z, bbar_x, bbar_y, bbar_z][n_points] (spectrum.hip step_prep)
+  const double *pre;
   int64_t chunk_base, chunk_n, n_grid;
   // decomposition and workspace
   int32_t n_groups;
@@ -98,7 +100,6 @@ __global__ void __launch_bounds__(kMaxT) k_adjoint(const AParams p) {
   const int q = (int)((int64_t)blockIdx.x / p.chunk_n);
   const int64_t ip = p.chunk_base + lp;
   const double dt = p.dt;
-  const double dtInv = 1.0 / dt;
   const int n_om = (int)p.n_omega;
 
   const Particle pt = load_particle(p, ip);
@@ -184,13 +185,9 @@ __global__ void __launch_bounds__(kMaxT) k_adjoint(const AParams p) {
         xp = p.x[gs - 1]; yp = p.y[gs - 1]; zp = p.z[gs - 1];
       }
       // direction-independent part of the step (kernel_farfield.cl:74-83)
-      const double u0x = p.ux[gs], u0y = p.uy[gs], u0z = p.uz[gs];
-      const double u1x = p.ux[gs + 1], u1y = p.uy[gs + 1], u1z = p.uz[gs + 1];
-      const double g0 = gamma_inv(u0x, u0y, u0z), g1 = gamma_inv(u1x, u1y, u1z);
-      const double b0x = u0x * g0, b0y = u0y * g0, b0z = u0z * g0;
-      const double b1x = u1x * g1, b1y = u1y * g1, b1z = u1z * g1;
-      const double a[3] = {(b1x - b0x) * dtInv, (b1y - b0y) * dtInv, (b1z - b0z) * dtInv};
-      const double m[3] = {0.5 * (b1x + b0x), 0.5 * (b1y + b0y), 0.5 * (b1z + b0z)};
+      const int64_t np = p.n_points;
+      const double a[3] = {p.pre[gs], p.pre[np + gs], p.pre[2 * np + gs]};
+      const double m[3] = {p.pre[3 * np + gs], p.pre[4 * np + gs], p.pre[5 * np + gs]};
       double acc9[kSlabWords];
 #pragma unroll
       for (int k = 0; k < kSlabWords; k++) acc9[k] = 0.0;
@@ -418,9 +415,10 @@ size_t adjoint_slab_words(const sr_tracks &t, const APlan &pl) {
   return (size_t)pl.groups * kSlabWords * (size_t)t.n_points + (size_t)pl.groups * (size_t)t.n_particles;
 }
 
+// workspace: slabs, the fields of one chunk of particles, the step data
 size_t adjoint_ws(const sr_tracks &t, const sr_grid &g, const APlan &pl) {
   const size_t n_grid = (size_t)g.n_omega * g.n_theta * g.n_phi;
-  return (adjoint_slab_words(t, pl) + 6 * (size_t)pl.chunk * n_grid) * sizeof(double);
+  return (adjoint_slab_words(t, pl) + 6 * (size_t)pl.chunk * n_grid + 6 * (size_t)t.n_points) * sizeof(double);
 }
 
 }  // namespace
@@ -477,15 +475,18 @@ int sr_far_adjoint_batch(const sr_tracks *t, const sr_grid *g, const sr_opts *o,
   p.slab_w = p.slab + (size_t)pl.groups * kSlabWords * (size_t)t->n_points;
   p.gx = gx; p.gy = gy; p.gz = gz; p.gux = gux; p.guy = guy; p.guz = guz; p.gw = gw;
   double *fields = (double *)workspace + adjoint_slab_words(*t, pl);
+  double *pre = fields + 6 * (size_t)pl.chunk * (size_t)p.n_grid;
   p.F = fields;
+  p.pre = pre;
   hipStream_t st = (hipStream_t)stream;
   const int64_t n_clear = (int64_t)adjoint_slab_words(*t, pl);
   hipLaunchKernelGGL(k_adjoint_clear, dim3((unsigned)((n_clear + 255) / 256)), dim3(256), 0, st, p.slab, n_clear);
+  if ((rc = step_prep(*t, SR_MODE_FAR, pre, st))) return rc;
   for (int64_t pb = 0; pb < t->n_particles; pb += pl.chunk) {
     const int64_t pe = std::min<int64_t>(t->n_particles, pb + pl.chunk);
     const int64_t n_f = 6 * (pe - pb) * p.n_grid;
     hipLaunchKernelGGL(k_adjoint_clear, dim3((unsigned)((n_f + 255) / 256)), dim3(256), 0, st, fields, n_f);
-    if ((rc = spectrum_fields(*t, *g, o->precision, pb, pe, fields, st))) return rc;
+    if ((rc = spectrum_fields(*t, *g, o->precision, pb, pe, pre, fields, st))) return rc;
     p.chunk_base = pb;
     p.chunk_n = pe - pb;
     const int64_t blocks = p.chunk_n * (int64_t)pl.groups;

@@ -30,11 +30,17 @@ const char *last_error();
 // x re, x im, y re, y im, z re, z im.
 constexpr int kCompField = 100;
 
+// The direction-independent step data of every sample into pre (far
+// [a_x, a_y, a_z, bbar_x, bbar_y, bbar_z][n_points], near [b_x, b_y, b_z]
+// [n_points]), as the tile kernel's stage would compute it (spectrum.hip).
+int step_prep(const sr_tracks &t, int32_t mode, double *pre, hipStream_t st);
+
 // Adds the fields of particles [p_begin, p_end) (one snapshot record each) into
 // F[6][p_end - p_begin][n_grid] with the tile kernel's own plan for `precision`
-// (spectrum.hip).  F must be cleared by the caller; stream-ordered.
+// (spectrum.hip); pre is step_prep's output or nullptr.  F must be cleared by
+// the caller; stream-ordered.
 int spectrum_fields(const sr_tracks &t, const sr_grid &g, int32_t precision, int64_t p_begin, int64_t p_end,
-                    double *F, hipStream_t st);
+                    const double *pre, double *F, hipStream_t st);
 
 // ---------------------------------------------------------------------------
 // device helpers
@@ -112,6 +118,21 @@ __device__ __forceinline__ void sincos_rev(double rv, float *s, float *c) {
 // 1/sqrt(1 + u.u) as the reference's rsqrt(1 + dot(u,u)), kernel_farfield.cl:77
 __device__ __forceinline__ double gamma_inv(double ux, double uy, double uz) {
   return 1.0 / sqrt(1.0 + (ux * ux + uy * uy + uz * uz));
+}
+
+// a = (b_{i+1} - b_i)/dt and bbar = (b_{i+1} + b_i)/2 of sample i's interval,
+// b = u rsqrt(1 + u.u), in the reference's expression order
+// (kernel_farfield.cl:74-83)
+__device__ __forceinline__ void step_am(const double *ux, const double *uy, const double *uz, int64_t i,
+                                        double dtInv, double *ax, double *ay, double *az, double *mx,
+                                        double *my, double *mz) {
+  const double u0x = ux[i], u0y = uy[i], u0z = uz[i];
+  const double u1x = ux[i + 1], u1y = uy[i + 1], u1z = uz[i + 1];
+  const double g0 = gamma_inv(u0x, u0y, u0z), g1 = gamma_inv(u1x, u1y, u1z);
+  const double b0x = u0x * g0, b0y = u0y * g0, b0z = u0z * g0;
+  const double b1x = u1x * g1, b1y = u1y * g1, b1z = u1z * g1;
+  *ax = (b1x - b0x) * dtInv; *ay = (b1y - b0y) * dtInv; *az = (b1z - b0z) * dtInv;
+  *mx = 0.5 * (b1x + b0x); *my = 0.5 * (b1y + b0y); *mz = 0.5 * (b1z + b0z);
 }
 
 struct Particle {

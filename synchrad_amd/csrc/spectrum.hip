@@ -116,6 +116,11 @@ struct KParams {
   // per-particle outputs (the adjoint's fields): particle ip writes at
   // (ip - part_base) * part_stride past the usual index; 0 otherwise
   int64_t part_base, part_stride;
+  // direction-independent step data from k_step_prep, or nullptr (the stage
+  // then computes it): far [a_x, a_y, a_z, bbar_x, bbar_y, bbar_z][n_points],
+  // near [b_x, b_y, b_z][n_points]
+  const double *pre;
+  int64_t n_points;
   // tiling
   uint32_t n_chunks;
   uint32_t n_dir;
@@ -635,9 +640,6 @@ k_spectrum(const KParams p) {
           const int64_t gs = pt.off + s;
           const double time = (double)(uint32_t)(pt.its + (uint32_t)s) * dt;
           const double xs = p.x[gs], ys = p.y[gs], zs = p.z[gs];
-          const double u0x = p.ux[gs], u0y = p.uy[gs], u0z = p.uz[gs];
-          const double g0 = gamma_inv(u0x, u0y, u0z);
-          const double b0x = u0x * g0, b0y = u0y * g0, b0z = u0z * g0;
           double tau, tau_prev = 0.0;
           if (MODE == SR_MODE_FAR) {
             // kernel_farfield.cl:65-94
@@ -646,11 +648,14 @@ k_spectrum(const KParams p) {
               const double tp = (double)(uint32_t)(pt.its + (uint32_t)(s - 1)) * dt;
               tau_prev = tp - (p.x[gs - 1] * gx + p.y[gs - 1] * gy + p.z[gs - 1] * gz);
             }
-            const double u1x = p.ux[gs + 1], u1y = p.uy[gs + 1], u1z = p.uz[gs + 1];
-            const double g1 = gamma_inv(u1x, u1y, u1z);
-            const double b1x = u1x * g1, b1y = u1y * g1, b1z = u1z * g1;
-            const double ax = (b1x - b0x) * dtInv, ay = (b1y - b0y) * dtInv, az = (b1z - b0z) * dtInv;
-            const double mx = 0.5 * (b1x + b0x), my = 0.5 * (b1y + b0y), mz = 0.5 * (b1z + b0z);
+            double ax, ay, az, mx, my, mz;
+            if (p.pre) {
+              const int64_t np = p.n_points;
+              ax = p.pre[gs]; ay = p.pre[np + gs]; az = p.pre[2 * np + gs];
+              mx = p.pre[3 * np + gs]; my = p.pre[4 * np + gs]; mz = p.pre[5 * np + gs];
+            } else {
+              step_am(p.ux, p.uy, p.uz, gs, dtInv, &ax, &ay, &az, &mx, &my, &mz);
+            }
             double c1 = ax * gx + ay * gy + az * gz;
             double c2 = 1.0 - (mx * gx + my * gy + mz * gz);
             c2 = 1.0 / c2;
@@ -676,6 +681,14 @@ k_spectrum(const KParams p) {
             }
           } else {
             // kernel_nearfield.cl:67-88; omega of c1 is applied at the record
+            double b0x, b0y, b0z;
+            if (p.pre) {
+              const int64_t np = p.n_points;
+              b0x = p.pre[gs]; b0y = p.pre[np + gs]; b0z = p.pre[2 * np + gs];
+            } else {
+              const double g0 = gamma_inv(p.ux[gs], p.uy[gs], p.uz[gs]);
+              b0x = p.ux[gs] * g0; b0y = p.uy[gs] * g0; b0z = p.uz[gs] * g0;
+            }
             const double rx = gx - xs, ry = gy - ys, rz = gz - zs;
             const double R = sqrt(rx * rx + ry * ry + rz * rz);
             tau = time + R;
@@ -817,6 +830,26 @@ k_spectrum(const KParams p) {
   }
 }
 
+
+// Direction-independent data of every sample (the stage's inputs that do not
+// depend on the spectral point), computed once per launch: far a, bbar of the
+// interval (i, i+1) (kernel_farfield.cl:74-83); near b_i (kernel_nearfield.cl:
+// 79-82).  The expressions are the stage's own, so the results are identical.
+template <int MODE>
+__global__ void k_step_prep(const double *ux, const double *uy, const double *uz, int64_t n, double dt,
+                            double *pre) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  if (MODE == SR_MODE_FAR) {
+    double ax = 0.0, ay = 0.0, az = 0.0, mx = 0.0, my = 0.0, mz = 0.0;
+    if (i + 1 < n) step_am(ux, uy, uz, i, 1.0 / dt, &ax, &ay, &az, &mx, &my, &mz);
+    pre[i] = ax; pre[n + i] = ay; pre[2 * n + i] = az;
+    pre[3 * n + i] = mx; pre[4 * n + i] = my; pre[5 * n + i] = mz;
+  } else {
+    const double g0 = gamma_inv(ux[i], uy[i], uz[i]);
+    pre[i] = ux[i] * g0; pre[n + i] = uy[i] * g0; pre[2 * n + i] = uz[i] * g0;
+  }
+}
 
 // out[c][i] += sum_q scratch[q][c][i], q in order
 __global__ void k_reduce_slices(const KParams p) {
@@ -1030,12 +1063,19 @@ void slice_particles(KParams &kp, int slices, int64_t n_particles, const int64_t
   kp.slice_lo[slices] = n_particles;
 }
 
-size_t workspace_for(const Plan &pl, int n_out, int64_t n_grid, uint32_t n_snaps) {
+// workspace of a batch call: the slices' partial spectra, then the
+// direction-independent step data
+size_t scratch_bytes(const Plan &pl, int n_out, int64_t n_grid, uint32_t n_snaps) {
   return pl.slices > 1 ? (size_t)pl.slices * n_out * (size_t)n_grid * n_snaps * sizeof(double) : 0;
 }
+size_t pre_bytes(int mode, int64_t n_points) {
+  return (size_t)(mode == SR_MODE_FAR ? 6 : 3) * (size_t)n_points * sizeof(double);
+}
 
+// want_pre: compute the step data into the workspace (batch calls); the
+// single-particle entry points have no workspace and compute it in the stage
 int run(KParams &kp, const Plan &pl, const sr_grid &g, const sr_opts &o, void *ws, size_t ws_bytes,
-        hipStream_t st) {
+        hipStream_t st, bool want_pre = false) {
   const int threads = kWave * pl.waves;
   const int64_t run = (int64_t)threads * pl.M;
   kp.n_chunks = (uint32_t)((g.n_omega + run - 1) / run);
@@ -1043,11 +1083,24 @@ int run(KParams &kp, const Plan &pl, const sr_grid &g, const sr_opts &o, void *w
   kp.n_tiles = pl.blocks;
   kp.acc_lds = (kp.n_out == 1 && kp.n_snaps == 1) ? 1 : 0;
   // per-particle outputs are disjoint: slices write them in place
-  const size_t need = kp.part_stride ? 0 : workspace_for(pl, kp.n_out, kp.n_grid, kp.n_snaps);
+  const size_t need = kp.part_stride ? 0 : scratch_bytes(pl, kp.n_out, kp.n_grid, kp.n_snaps);
+  const size_t need_pre = want_pre ? pre_bytes(g.mode, kp.n_points) : 0;
   hipError_t e;
+  if (need + need_pre && (!ws || ws_bytes < need + need_pre))
+    return fail(SR_ERR_WORKSPACE, "workspace of %zu bytes needed, %zu given", need + need_pre, ws_bytes);
+  if (need_pre) {
+    kp.pre = (const double *)((char *)ws + need);
+    const int64_t n = kp.n_points;
+    if (g.mode == SR_MODE_FAR)
+      hipLaunchKernelGGL(k_step_prep<SR_MODE_FAR>, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, kp.ux,
+                         kp.uy, kp.uz, n, kp.dt, (double *)kp.pre);
+    else
+      hipLaunchKernelGGL(k_step_prep<SR_MODE_NEAR>, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, kp.ux,
+                         kp.uy, kp.uz, n, kp.dt, (double *)kp.pre);
+    e = hipGetLastError();
+    if (e != hipSuccess) return fail(SR_ERR_HIP, "step data launch: %s", hipGetErrorString(e));
+  }
   if (need) {
-    if (!ws || ws_bytes < need)
-      return fail(SR_ERR_WORKSPACE, "workspace of %zu bytes needed, %zu given", need, ws_bytes);
     kp.scratch = (double *)ws;
     const int64_t n = (int64_t)(need / sizeof(double));
     hipLaunchKernelGGL(k_fill_zero, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, kp.scratch, n);
@@ -1128,8 +1181,20 @@ int single_particle(int32_t mode, int32_t comp, double *const *outs, const doubl
 
 namespace sr {
 
+int step_prep(const sr_tracks &t, int32_t mode, double *pre, hipStream_t st) {
+  const int64_t n = t.n_points;
+  if (mode == SR_MODE_FAR)
+    hipLaunchKernelGGL(k_step_prep<SR_MODE_FAR>, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, t.ux, t.uy,
+                       t.uz, n, t.dt, pre);
+  else
+    hipLaunchKernelGGL(k_step_prep<SR_MODE_NEAR>, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, t.ux, t.uy,
+                       t.uz, n, t.dt, pre);
+  const hipError_t e = hipGetLastError();
+  return e == hipSuccess ? SR_OK : fail(SR_ERR_HIP, "step data launch: %s", hipGetErrorString(e));
+}
+
 int spectrum_fields(const sr_tracks &t, const sr_grid &g, int32_t precision, int64_t p_begin, int64_t p_end,
-                    double *F, hipStream_t st) {
+                    const double *pre, double *F, hipStream_t st) {
   KParams kp;
   memset(&kp, 0, sizeof(kp));
   kp.offsets = t.offsets;
@@ -1141,6 +1206,8 @@ int spectrum_fields(const sr_tracks &t, const sr_grid &g, int32_t precision, int
   kp.n_snaps = 1;
   kp.snaps_stride = t.snaps_stride;
   kp.dt = t.dt;
+  kp.n_points = t.n_points;
+  kp.pre = pre;
   sr_opts o;
   memset(&o, 0, sizeof(o));
   o.comp = kCompField;
@@ -1179,9 +1246,11 @@ int sr_workspace_bytes(const sr_tracks *tracks, const sr_grid *grid, const sr_op
   if (!bytes) return fail(SR_ERR_INVALID, "bytes is NULL");
   *bytes = 0;
   if ((rc = check_tracks(tracks)) || (rc = check_grid(grid)) || (rc = check_opts(grid, opts))) return rc;
+  if (tracks->n_particles == 0) return SR_OK;
   const Plan pl = make_plan(*grid, *opts, tracks->n_particles, tracks->n_snaps);
-  *bytes = workspace_for(pl, outputs_for(grid->mode, opts->comp),
-                         (int64_t)grid->n_omega * grid->n_theta * grid->n_phi, tracks->n_snaps);
+  *bytes = scratch_bytes(pl, outputs_for(grid->mode, opts->comp),
+                         (int64_t)grid->n_omega * grid->n_theta * grid->n_phi, tracks->n_snaps) +
+           pre_bytes(grid->mode, tracks->n_points);
   return SR_OK;
 }
 
@@ -1205,10 +1274,11 @@ int sr_spectrum_batch(const sr_tracks *t, const sr_grid *g, const sr_opts *o, do
   kp.n_snaps = t->n_snaps;
   kp.snaps_stride = t->snaps_stride;
   kp.dt = t->dt;
+  kp.n_points = t->n_points;
   fill_grid(kp, *g, *o, n_out, outs);
   const Plan pl = make_plan(*g, *o, t->n_particles, t->n_snaps);
   slice_particles(kp, pl.slices, t->n_particles, t->offsets_host);
-  return run(kp, pl, *g, *o, workspace, workspace_bytes, (hipStream_t)stream);
+  return run(kp, pl, *g, *o, workspace, workspace_bytes, (hipStream_t)stream, true);
 }
 
 int sr_far_total(double *spectrum, const double *x, const double *y, const double *z,

@@ -132,20 +132,29 @@ def test_empty_batch_is_a_no_op():
 
 
 def test_workspace_plan():
-    # large grid, one snapshot: no particle slices, no workspace
-    rc, n = _ws(_tracks(n_particles=2000), _grid(), _opts())
-    assert rc == 0 and n == 0
-    # small grid: particle slices need n_slices * n_out * n_grid doubles
+    # every batch holds the direction-independent step data: 6 doubles per
+    # sample in the far field (a, bbar), 3 in the near field (b)
+    t = _tracks(n_particles=2000)
+    pre = 6 * 8 * t.n_points
+    # large grid, one snapshot: no particle slices, only the step data
+    rc, n = _ws(t, _grid(), _opts())
+    assert rc == 0 and n == pre
+    # small grid: particle slices add n_slices * n_out * n_grid doubles
     g = _grid(n_omega=128, n_theta=32, n_phi=32)
-    rc, n = _ws(_tracks(n_particles=256), g, _opts())
-    assert rc == 0 and n > 0 and n % (8 * 128 * 32 * 32) == 0
+    t256 = _tracks(n_particles=256)
+    pre256 = 6 * 8 * t256.n_points
+    rc, n = _ws(t256, g, _opts())
+    assert rc == 0 and n > pre256 and (n - pre256) % (8 * 128 * 32 * 32) == 0
     # several snapshots: the slices hold every snapshot
-    rc, n1 = _ws(_tracks(n_particles=256), g, _opts())
+    rc, n1 = _ws(t256, g, _opts())
     rc, n3 = _ws(_tracks(n_particles=256, n_snaps=3, snaps_stride=3), g, _opts())
-    assert rc == 0 and n3 == 3 * n1
+    assert rc == 0 and n3 - pre256 == 3 * (n1 - pre256)
     # explicit slices
-    rc, n = _ws(_tracks(n_particles=256), g, _opts(slices=3, comp=_abi.SR_COMP_CARTESIAN))
-    assert rc == 0 and n == 3 * 3 * 128 * 32 * 32 * 8
+    rc, n = _ws(t256, g, _opts(slices=3, comp=_abi.SR_COMP_CARTESIAN))
+    assert rc == 0 and n == 3 * 3 * 128 * 32 * 32 * 8 + pre256
+    # empty batch: nothing
+    rc, n = _ws(_tracks(n_particles=0), g, _opts())
+    assert rc == 0 and n == 0
 
 
 def test_per_particle_entry_points_validate():
