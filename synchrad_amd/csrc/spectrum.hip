@@ -14,36 +14,45 @@
 // How it is laid out on gfx950:
 //   * A block owns one direction (far) or screen point (near) and a run of
 //     T*M omegas: each of the T = 64*waves threads holds M omegas (the k-th
-//     omega of thread t is chunk*T*M + k*T + t, so every output access of a
-//     wave covers 64 consecutive doubles).  A tile index also carries a
-//     particle slice q; the slices of one tile are summed in slice order by a
-//     second kernel.
+//     omega of thread t is chunk*T*M + k*T + t', t' = t with the 64-omega runs
+//     dealt to the waves in an order rotated by the tile index, so every
+//     output access of a wave covers 64 consecutive doubles).  A tile index
+//     also carries a particle slice q; the slices of one tile are summed in
+//     slice order by a second kernel.
+//   * A pre-pass (k_step_prep) computes the direction-independent data of
+//     every sample once per call: far a and the mean velocity of the interval,
+//     near the velocity.
 //   * The block walks its particles (CSR) and their steps in tiles of T steps.
 //     Stage: thread t computes, in FP64, everything of step t0+t that does not
 //     depend on omega -- tau = t - n.x (far) or t + |P - x| (near) and the
-//     amplitude vector(s) -- and stores it in LDS.  Sweep: every thread reads
+//     amplitude words -- and stores it in LDS.  Sweep: every thread reads
 //     the tile back with broadcast LDS reads and accumulates its omegas.  The
 //     omega-independent work is paid once per (point, step) and shared by the
 //     block's T*M omegas.
+//   * Amplitude words: far split A.e_theta, A.e_phi (A is perpendicular to n);
+//     far FP64 A_x, A_y, A_z; near v1 = (b - n)/R and v2 = n/R^2.
 //   * Sweeps:
 //       F64 direct    FP64 phase omega*tau, |dphase| < pi filter exactly as the
-//                     reference forms it, two-constant Cody-Waite reduction,
-//                     FP64 sincos, FP64 sums.
-//       split direct  same FP64 phase and filter; the reduced phase in
-//                     revolutions feeds v_sin_f32/v_cos_f32; FP32 amplitudes
-//                     and sums with a Kahan fold per swept segment.
-//       split rec     for omega grids with a constant step (linear grids): the
-//                     thread's M omegas differ by T*domega, so exp(i omega_k tau)
-//                     is advanced from k to k+1 by one complex multiply with
-//                     exp(i T domega tau), which the stage computes per step
-//                     (FP64 phase, FP64 reduction, FP32 sin/cos).  The filter
-//                     becomes the per-step threshold omega < pi/|tau_s - tau_{s-1}|
-//                     (the same inequality with omega factored out), and whole
-//                     steps are skipped or taken unmasked when the block's omega
-//                     range lies entirely on one side of it.
-//   * Outputs are read-modify-written once per particle record by the one
-//     thread that owns them: no atomics, deterministic results, and the
-//     particle order of the reference's sequential `spectrum +=` is kept.
+//                     reference forms it, FP64 sin/cos (quadrant reduction and
+//                     Taylor polynomials), FP64 sums; 1 or 2 omegas per thread.
+//       split direct  (grids without a constant step) FP64 phase in
+//                     revolutions -> v_fract_f64 -> v_sin_f32/v_cos_f32, the
+//                     filter in its threshold form |omega|/2pi < 1/(2|dtau|),
+//                     FP32 sums with a Kahan fold per swept segment.
+//       split rec     (linear grids) the thread's M omegas differ by T*domega,
+//                     so exp(i omega_k tau) is advanced from k to k+1 by one
+//                     complex multiply with exp(i T domega tau) (computed per
+//                     step in the stage).  The tile's steps are put in an order
+//                     in which every 64-omega run finds the steps keeping any of
+//                     its omegas first (a stable counting sort), so each wave
+//                     sweeps only those, stops the rotation chain at its last
+//                     kept run and masks only that run.
+//   * Records: with one snapshot and one output each thread sums its points
+//     over the block's particles in LDS and writes once; otherwise outputs are
+//     read-modify-written per particle record by the one thread that owns
+//     them.  No atomics anywhere, so results do not depend on scheduling.
+//   * An internal output kind writes each particle's complex field separately
+//     (the adjoint's forward pass).
 //
 // All code here is compiled with -ffp-contract=off: the FP64 stage reproduces
 // the reference's expression order term by term, and fused multiply-adds are
