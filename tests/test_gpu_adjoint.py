@@ -346,3 +346,31 @@ def test_adjoint_config5_fd_50_coordinates():
               f"FD {est:+.10e} rel {err:.2e} (h0 {h0:.3e})", flush=True)
         assert err < 1e-6, (i, p, c, s, est, got)
     print(f"config5 FP64 adjoint vs FP64 central FD, 50 coordinates: worst {worst:.2e}", flush=True)
+
+
+@pytest.mark.parametrize("precision,tol", [("double", 1e-10), ("float", 1e-5)])
+def test_adjoint_chunks_of_particles(precision, tol):
+    """On a 512 x 64 x 64 grid one particle's fields take ~100 MB, so 30
+    particles are processed in two chunks of the ~2 GiB field workspace.
+    Each sample's gradient depends only on its own particle's field, so the
+    batch must agree with the particles run one at a time (to the FP64
+    rounding of the regrouped sums; the split path's own rounding)."""
+    wl = workloads.config2(n_particles=30, n_steps=200)
+    ax = wl.axes
+    grid = engine.DeviceGrid("far", ax["omega"], ax["theta"], ax["phi"])
+    hb = wl.batch
+    rng = np.random.default_rng(7)
+    G = torch.from_numpy(rng.standard_normal((1, ax["phi"].size, ax["theta"].size, ax["omega"].size))).cuda()
+    full = engine.far_adjoint(engine.DeviceBatch(hb), grid, [G], precision=precision)
+    torch.cuda.synchronize()
+    full = [t.cpu().numpy() for t in full]
+    one = [[] for _ in range(7)]
+    for t in wl.track_list():
+        g = engine.far_adjoint(engine.DeviceBatch(engine.pack_tracks([t], dt=hb.dt)), grid, [G], precision=precision)
+        torch.cuda.synchronize()
+        for c in range(7):
+            one[c].append(g[c].cpu().numpy())
+    for c in range(7):
+        r = rel_l2(full[c], np.concatenate(one[c]))
+        print(f"chunked adjoint {precision} grad[{c}] vs one particle at a time: rel-L2 {r:.3e}", flush=True)
+        assert r < tol
