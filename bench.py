@@ -13,7 +13,11 @@ Prints one JSON line.  ``roofline`` prices the tile kernel at the reference
 integrand's algorithmic cost (49 flop per far-field evaluation, 54 near)
 against MI355X's FP32 vector peak; ``cpu_baseline`` times the CPU FP64 oracle
 (a C restatement of the reference kernels, not the reference) on a bounded
-share of the same workload on this host.
+share of the same workload on this host, and ``accuracy_vs_oracle`` compares
+both GPU paths with that oracle result (relative L2 and largest error).
+``end_to_end`` times one drop-in ``SynchRad.calculate_spectrum`` from host
+track lists (packing, uploads, kernel, download); ``value`` is always the
+HBM-resident rate.
 
 Other configurations (--config 1..5) print their own line: 1 undulator (FP64),
 3 uneven CSR batch (cartesian; also one launch per particle on a share),
@@ -57,6 +61,7 @@ def parse():
     ap.add_argument("--no-f64", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-autotune", action="store_true")
+    ap.add_argument("--no-e2e", action="store_true", help="skip the drop-in call from host arrays")
     ap.add_argument("--cpu-particles", type=int, default=4)
     ap.add_argument("--cpu-grid", type=str, default="128,32,16")
     ap.add_argument("--per-particle-share", type=int, default=400,
@@ -121,6 +126,14 @@ def rel_l2_dev(a_list, b_list):
     return (num / den) ** 0.5 if den > 0 else float("nan")
 
 
+def max_err_dev(a_list, b_list):
+    """Largest elementwise |a - b| relative to the largest |b|."""
+    import torch
+    num = max(float(torch.max(torch.abs(a - b))) for a, b in zip(a_list, b_list))
+    den = max(float(torch.max(torch.abs(b))) for b in b_list)
+    return num / den if den > 0 else float("nan")
+
+
 def active_fraction(wl, n_particles):
     """Share of point-step evaluations that pass the reference's filter
     |omega (tau_s - tau_{s-1})| < pi (far field, phase_{-1} = 0), counted on
@@ -178,12 +191,58 @@ def cpu_baseline(wl_full, cfg, n_particles, grid_str):
     evals = ax["omega"].size * n1 * ax["phi"].size * sum(max(t[0].size - 1, 0) for t in tr)
     threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
     t0 = time.perf_counter()
-    oracle.spectrum(wl.mode, wl.comp, tr, ax, wl.batch.dt, L_screen=wl.L_screen, threads=threads)
+    ref = oracle.spectrum(wl.mode, wl.comp, tr, ax, wl.batch.dt, L_screen=wl.L_screen, threads=threads)
     sec = time.perf_counter() - t0
-    return {"value": round(evals / sec / 1e9, 4), "unit": "G evals/s", "cores": threads, "kind": "port",
-            "sample": f"{len(tr)} particles x {tr[0][0].size} steps on a {sample_grid} grid "
-                      f"({evals:.3g} evals, {sec:.1f} s), FP64 C/OpenMP restatement of the "
-                      f"reference kernels"}
+    desc = f"{len(tr)} particles x {tr[0][0].size} steps on a {sample_grid} grid"
+    res = {"value": round(evals / sec / 1e9, 4), "unit": "G evals/s", "cores": threads, "kind": "port",
+           "sample": f"{desc} ({evals:.3g} evals, {sec:.1f} s), FP64 C/OpenMP restatement of the "
+                     f"reference kernels"}
+    return res, (wl, tr, ref, desc)
+
+
+OUT_KEYS = {"total": ["total"], "cartesian": ["x", "y", "z"]}
+
+
+def accuracy_vs_oracle(sample):
+    """Both GPU paths against the CPU oracle's FP64 result on the cpu_baseline
+    sample: relative L2 and the largest elementwise error relative to the
+    largest reference value."""
+    import torch
+    from synchrad_amd import engine
+    wl, tr, ref, desc = sample
+    ax = wl.axes
+    ax1 = ax["theta"] if wl.mode == "far" else ax["radius"]
+    grid = engine.DeviceGrid(wl.mode, ax["omega"], ax1, ax["phi"], L_screen=wl.L_screen)
+    batch = engine.DeviceBatch(engine.pack_tracks(tr, dt=wl.batch.dt))
+    keys = OUT_KEYS[wl.comp]
+    r = np.stack([ref[k] for k in keys])
+    out = {"sample": desc}
+    for prec in ("double", "float"):
+        outs = engine.spectrum(batch, grid, comp=wl.comp, precision=prec)
+        torch.cuda.synchronize()
+        g = np.stack([o.cpu().numpy().swapaxes(-1, -3) for o in outs])
+        out["f64" if prec == "double" else "split"] = {
+            "rel_l2": float(np.linalg.norm(g - r) / np.linalg.norm(r)),
+            "max_err_rel_to_max": float(np.abs(g - r).max() / np.abs(r).max())}
+    return out
+
+
+def end_to_end(wl, precision):
+    """One drop-in SynchRad.calculate_spectrum from host track lists: CSR
+    packing, uploads, the library call and the download of the spectrum."""
+    import torch
+    from synchrad_amd.calc import SynchRad
+    calc = SynchRad(dict(wl.calc_input, dtype=precision))
+    tracks = wl.track_list()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    calc.calculate_spectrum(particleTracks=tracks, timeStep=wl.batch.dt, comp=wl.comp, L_screen=wl.L_screen,
+                            verbose=False)
+    torch.cuda.synchronize()
+    ms = (time.perf_counter() - t0) * 1e3
+    return {"value": round(wl.evals / (ms * 1e-3) / 1e9, 3), "unit": "G evals/s", "ms": round(ms, 1),
+            "path": "SynchRad.calculate_spectrum from host track lists (CSR packing, H2D, library call, "
+                    "D2H), library's own plan"}
 
 
 def plan_for(batch, grid, comp, precision, args):
@@ -264,19 +323,28 @@ def bench_forward(args):
         v64 = ev64 / (ms64 * 1e-3) / 1e9
         _, _, _, outs_s = time_spectra(b64, g64, wl.comp, "float", 1, 0, plan)
         rel = rel_l2_dev(outs_s, outs64)
+        mx = max_err_dev(outs_s, outs64)
         log(f"FP64: {ms64:.1f} ms ({v64:.1f} G evals/s); split vs FP64 rel-L2 {rel:.3e}")
         res["f64"] = {"value": round(v64, 3), "unit": "G evals/s", "particles": n64,
                       "frac_fp64_peak": round(v64 * 1e9 * flop / 1e12 / PEAK_FP64_TFLOPS, 4)}
         res["split_over_f64"] = round(value / v64, 3)
         res["rel_l2_split_vs_f64"] = rel
+        res["max_err_split_vs_f64"] = mx
 
     if cfg == 3 and args.per_particle_share > 0:
         res["per_particle_launches"] = per_particle_compare(wl, grid, plan, args.per_particle_share)
 
+    if not args.no_e2e:
+        log("timing the drop-in call from host arrays")
+        res["end_to_end"] = end_to_end(wl, main_prec)
+        log(f"end to end: {res['end_to_end']}")
+
     if not args.no_cpu:
         log("timing CPU oracle sample")
-        res["cpu_baseline"] = cpu_baseline(wl, cfg, args.cpu_particles, args.cpu_grid)
+        res["cpu_baseline"], sample = cpu_baseline(wl, cfg, args.cpu_particles, args.cpu_grid)
         log(f"cpu: {res['cpu_baseline']}")
+        res["accuracy_vs_oracle"] = accuracy_vs_oracle(sample)
+        log(f"accuracy: {res['accuracy_vs_oracle']}")
     return res
 
 
