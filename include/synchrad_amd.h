@@ -178,6 +178,37 @@ int sr_far_adjoint_batch(const sr_tracks *tracks, const sr_grid *grid, const sr_
                          void *workspace, size_t workspace_bytes, void *stream);
 
 /*
+ * The forward half of the adjoint, kept for its backward: every particle's
+ * complex far field F_p(g) = sum_s A_s e^{i phase_s} (no weight, no dt), as
+ * fields[6][n_particles][n_grid] doubles (x, y, z as re/im pairs, grid index
+ * as in the spectra), and, when `outs` is not NULL, the spectra of comp
+ * SR_COMP_TOTAL (1 array) or SR_COMP_CARTESIAN (3) ADDED from those fields
+ * (w dt^2 |F_p|^2 particle by particle in particle order, the records'
+ * expressions of synchrad/kernel_farfield.cl:100-106, 208-222).  No
+ * counterpart in the reference.  Same restrictions as the adjoint (far field,
+ * one snapshot); the workspace holds the step data (6 * n_points doubles).
+ */
+int sr_far_fields_workspace_bytes(const sr_tracks *tracks, const sr_grid *grid, const sr_opts *opts,
+                                  size_t *bytes);
+int sr_far_fields_batch(const sr_tracks *tracks, const sr_grid *grid, const sr_opts *opts,
+                        double *fields, double *const *outs,
+                        void *workspace, size_t workspace_bytes, void *stream);
+
+/*
+ * sr_far_adjoint_batch with the fields of sr_far_fields_batch (same tracks,
+ * grid and precision) passed in instead of recomputed: the backward of a
+ * forward that kept them.  The workspace holds only the group slabs and the
+ * step data.  No counterpart in the reference.
+ */
+int sr_far_adjoint_fields_workspace_bytes(const sr_tracks *tracks, const sr_grid *grid,
+                                          const sr_opts *opts, size_t *bytes);
+int sr_far_adjoint_from_fields(const sr_tracks *tracks, const sr_grid *grid, const sr_opts *opts,
+                               const double *const *dS, const double *fields,
+                               double *gx, double *gy, double *gz,
+                               double *gux, double *guy, double *guz, double *gw,
+                               void *workspace, size_t workspace_bytes, void *stream);
+
+/*
  * Per-particle entry points with the argument order of the reference's
  * OpenCL kernels, plus a precision selector and a stream.  One call adds one
  * particle's contribution, as one reference launch does.

@@ -82,6 +82,8 @@ EXPORTS = (
     "sr_abi_version", "sr_last_error", "sr_num_outputs", "sr_workspace_bytes",
     "sr_spectrum_batch", "sr_far_total", "sr_far_cartesian", "sr_near_total",
     "sr_near_cartesian", "sr_far_adjoint_workspace_bytes", "sr_far_adjoint_batch",
+    "sr_far_fields_workspace_bytes", "sr_far_fields_batch", "sr_far_adjoint_fields_workspace_bytes",
+    "sr_far_adjoint_from_fields",
 )
 
 _lib = None
@@ -130,6 +132,18 @@ def load(path: str = LIB):
     L.sr_far_adjoint_batch.argtypes = [ctypes.POINTER(SrTracks), ctypes.POINTER(SrGrid),
                                        ctypes.POINTER(SrOpts), ctypes.POINTER(_vp)] + [_vp] * 7 + [
                                        _vp, ctypes.c_size_t, _vp]
+    L.sr_far_fields_workspace_bytes.restype = ctypes.c_int
+    L.sr_far_fields_workspace_bytes.argtypes = L.sr_workspace_bytes.argtypes
+    L.sr_far_fields_batch.restype = ctypes.c_int
+    L.sr_far_fields_batch.argtypes = [ctypes.POINTER(SrTracks), ctypes.POINTER(SrGrid),
+                                      ctypes.POINTER(SrOpts), _vp, ctypes.POINTER(_vp), _vp,
+                                      ctypes.c_size_t, _vp]
+    L.sr_far_adjoint_fields_workspace_bytes.restype = ctypes.c_int
+    L.sr_far_adjoint_fields_workspace_bytes.argtypes = L.sr_workspace_bytes.argtypes
+    L.sr_far_adjoint_from_fields.restype = ctypes.c_int
+    L.sr_far_adjoint_from_fields.argtypes = [ctypes.POINTER(SrTracks), ctypes.POINTER(SrGrid),
+                                             ctypes.POINTER(SrOpts), ctypes.POINTER(_vp), _vp] + \
+        [_vp] * 7 + [_vp, ctypes.c_size_t, _vp]
     v = L.sr_abi_version()
     if v != SR_ABI_VERSION:
         raise SynchradError(f"ABI version mismatch: library {v}, binding {SR_ABI_VERSION}")

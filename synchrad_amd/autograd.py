@@ -8,8 +8,12 @@ HIP forward and adjoint kernels.
 ``layout`` is an :class:`engine.HostBatch` giving the CSR offsets, time
 alignment and snapshot of the tracks (its own coordinate arrays are not used;
 the tensors passed in are).  The |dphase| < pi mask of the forward is held
-fixed in the backward; the adjoint recomputes every step tile instead of
-storing any intermediate.
+fixed in the backward.  When every particle's complex field fits
+``KEEP_FIELDS_BYTES`` (48 bytes per particle and spectral point), the forward
+computes those fields once, forms the spectrum from them and keeps them for
+the backward, which then runs only the transpose sweep; otherwise the forward
+runs the spectrum kernel and the backward recomputes the fields chunk by
+chunk.  No other intermediate is stored.
 """
 from __future__ import annotations
 
@@ -19,6 +23,9 @@ import torch
 
 from . import engine
 
+# largest per-particle field set the forward keeps for the backward (bytes)
+KEEP_FIELDS_BYTES = 8 << 30
+
 
 class FarSpectrumFunction(torch.autograd.Function):
     @staticmethod
@@ -26,14 +33,20 @@ class FarSpectrumFunction(torch.autograd.Function):
                 comp: str, precision: str, tuning: Optional[engine.Tuning]):
         ts = [t.detach().contiguous() for t in (x, y, z, ux, uy, uz, w)]
         batch = engine.DeviceBatch(layout, grid.device, tensors=ts)
-        outs = engine.spectrum(batch, grid, comp=comp, precision=precision, tuning=tuning)
+        ctx.fields = None
+        if 0 < engine.field_bytes(batch, grid) <= KEEP_FIELDS_BYTES:
+            ctx.fields, outs = engine.far_fields(batch, grid, comp=comp, precision=precision)
+        else:
+            outs = engine.spectrum(batch, grid, comp=comp, precision=precision, tuning=tuning)
         ctx.batch, ctx.grid, ctx.comp, ctx.precision = batch, grid, comp, precision
         return torch.stack(outs)
 
     @staticmethod
     def backward(ctx, gS):
         dS = [gS[c] for c in range(gS.shape[0])]
-        g = engine.far_adjoint(ctx.batch, ctx.grid, dS, comp=ctx.comp, precision=ctx.precision)
+        g = engine.far_adjoint(ctx.batch, ctx.grid, dS, comp=ctx.comp, precision=ctx.precision,
+                               fields=ctx.fields)
+        ctx.fields = None
         return (*g, None, None, None, None, None)
 
 

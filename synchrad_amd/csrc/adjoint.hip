@@ -349,6 +349,42 @@ __global__ void k_adjoint_clear(double *buf, int64_t n) {
   if (i < n) buf[i] = 0.0;
 }
 
+// Spectra from the per-particle fields F[6][n_particles][n_grid] (x, y, z
+// as re/im pairs): out += w dt^2 |F|^2 particle by particle, in particle order
+// and with the record's own expressions (spectrum.hip emit_k), so the sums
+// equal those of the spectrum kernel run without particle slices.
+__global__ void k_fields_spectrum(const int64_t *offsets, const double *weight, double dt, int64_t n_particles,
+                                  int64_t n_grid, int32_t comp, const double *F, double *out0, double *out1,
+                                  double *out2) {
+  const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= n_grid) return;
+  const int64_t fs = n_particles * n_grid;
+  double acc[3] = {out0[g], comp == SR_COMP_TOTAL ? 0.0 : out1[g], comp == SR_COMP_TOTAL ? 0.0 : out2[g]};
+  for (int64_t ip = 0; ip < n_particles; ip++) {
+    if (offsets[ip + 1] <= offsets[ip]) continue;   // no samples: no record
+    const double wpdt2 = weight[ip] * dt * dt;
+    const double *f = F + ip * n_grid + g;
+    double re[3], im[3];
+#pragma unroll
+    for (int c = 0; c < 3; c++) {
+      re[c] = f[(2 * c) * fs];
+      im[c] = f[(2 * c + 1) * fs];
+    }
+    if (comp == SR_COMP_TOTAL) {
+      acc[0] += wpdt2 * ((re[0] * re[0] + re[1] * re[1] + re[2] * re[2]) +
+                         (im[0] * im[0] + im[1] * im[1] + im[2] * im[2]));
+    } else {
+#pragma unroll
+      for (int c = 0; c < 3; c++) acc[c] += wpdt2 * (re[c] * re[c] + im[c] * im[c]);
+    }
+  }
+  out0[g] = acc[0];
+  if (comp != SR_COMP_TOTAL) {
+    out1[g] = acc[1];
+    out2[g] = acc[2];
+  }
+}
+
 struct APlan {
   int threads;
   int groups;
@@ -363,14 +399,17 @@ constexpr size_t kAdjTableBudget = 32768;
 // fields of one chunk of particles: 6 doubles per (particle, point), ~2 GiB at most
 constexpr double kFieldBudget = 2147483648.0;
 
-APlan adjoint_plan(const sr_tracks &t, const sr_grid &g, const sr_opts &o) {
+// all_fields: the caller holds every particle's fields (one chunk)
+APlan adjoint_plan(const sr_tracks &t, const sr_grid &g, const sr_opts &o, bool all_fields) {
   APlan pl;
   // threads = steps of the transpose sweep
   const int waves = o.waves_per_block > 0 ? o.waves_per_block : 4;
   pl.threads = kWave * waves;
   const int64_t n_dir = (int64_t)g.n_theta * g.n_phi;
   const int64_t n_grid = n_dir * g.n_omega;
-  pl.chunk = std::max<int64_t>(1, std::min<int64_t>(t.n_particles, (int64_t)(kFieldBudget / (48.0 * n_grid))));
+  pl.chunk = all_fields ? std::max<int64_t>(1, t.n_particles)
+                        : std::max<int64_t>(1, std::min<int64_t>(t.n_particles,
+                                                                 (int64_t)(kFieldBudget / (48.0 * n_grid))));
   int64_t groups = o.slices > 0 ? o.slices : std::max<int64_t>(1, (4096 + pl.chunk - 1) / pl.chunk);
   pl.groups = (int)std::min<int64_t>(std::min<int64_t>(groups, n_dir), 1 << 16);
   const bool split = o.precision == SR_PREC_SPLIT;
@@ -415,30 +454,19 @@ size_t adjoint_slab_words(const sr_tracks &t, const APlan &pl) {
   return (size_t)pl.groups * kSlabWords * (size_t)t.n_points + (size_t)pl.groups * (size_t)t.n_particles;
 }
 
-// workspace: slabs, the fields of one chunk of particles, the step data
-size_t adjoint_ws(const sr_tracks &t, const sr_grid &g, const APlan &pl) {
+// workspace: slabs, the fields of one chunk of particles (unless the caller
+// passes them), the step data
+size_t adjoint_ws(const sr_tracks &t, const sr_grid &g, const APlan &pl, bool own_fields) {
   const size_t n_grid = (size_t)g.n_omega * g.n_theta * g.n_phi;
-  return (adjoint_slab_words(t, pl) + 6 * (size_t)pl.chunk * n_grid + 6 * (size_t)t.n_points) * sizeof(double);
+  return (adjoint_slab_words(t, pl) + (own_fields ? 6 * (size_t)pl.chunk * n_grid : 0) +
+          6 * (size_t)t.n_points) * sizeof(double);
 }
 
-}  // namespace
-
-extern "C" {
-
-int sr_far_adjoint_workspace_bytes(const sr_tracks *tracks, const sr_grid *grid, const sr_opts *opts,
-                                   size_t *bytes) {
-  if (!bytes) return fail(SR_ERR_INVALID, "bytes is NULL");
-  *bytes = 0;
-  int rc = adjoint_check(tracks, grid, opts);
-  if (rc) return rc;
-  if (tracks->n_particles == 0) return SR_OK;
-  *bytes = adjoint_ws(*tracks, *grid, adjoint_plan(*tracks, *grid, *opts));
-  return SR_OK;
-}
-
-int sr_far_adjoint_batch(const sr_tracks *t, const sr_grid *g, const sr_opts *o, const double *const *dS,
-                         double *gx, double *gy, double *gz, double *gux, double *guy, double *guz,
-                         double *gw, void *workspace, size_t workspace_bytes, void *stream) {
+// the adjoint; fields_in = NULL: the fields are computed chunk by chunk in the
+// workspace, else they are the caller's F[6][n_particles][n_grid]
+int adjoint_run(const sr_tracks *t, const sr_grid *g, const sr_opts *o, const double *const *dS,
+                const double *fields_in, double *gx, double *gy, double *gz, double *gux, double *guy,
+                double *guz, double *gw, void *workspace, size_t workspace_bytes, void *stream) {
   int rc = adjoint_check(t, g, o);
   if (rc) return rc;
   if (t->n_particles == 0) return SR_OK;
@@ -447,8 +475,9 @@ int sr_far_adjoint_batch(const sr_tracks *t, const sr_grid *g, const sr_opts *o,
   for (int c = 0; c < n_in; c++)
     if (!dS[c]) return fail(SR_ERR_INVALID, "dS[%d] is NULL", c);
   if (!gx || !gy || !gz || !gux || !guy || !guz || !gw) return fail(SR_ERR_INVALID, "a gradient pointer is NULL");
-  const APlan pl = adjoint_plan(*t, *g, *o);
-  const size_t need = adjoint_ws(*t, *g, pl);
+  const bool own = (fields_in == nullptr);
+  const APlan pl = adjoint_plan(*t, *g, *o, !own);
+  const size_t need = adjoint_ws(*t, *g, pl, own);
   if (!workspace || workspace_bytes < need)
     return fail(SR_ERR_WORKSPACE, "workspace of %zu bytes needed, %zu given", need, workspace_bytes);
   AParams p;
@@ -475,8 +504,8 @@ int sr_far_adjoint_batch(const sr_tracks *t, const sr_grid *g, const sr_opts *o,
   p.slab_w = p.slab + (size_t)pl.groups * kSlabWords * (size_t)t->n_points;
   p.gx = gx; p.gy = gy; p.gz = gz; p.gux = gux; p.guy = guy; p.guz = guz; p.gw = gw;
   double *fields = (double *)workspace + adjoint_slab_words(*t, pl);
-  double *pre = fields + 6 * (size_t)pl.chunk * (size_t)p.n_grid;
-  p.F = fields;
+  double *pre = own ? fields + 6 * (size_t)pl.chunk * (size_t)p.n_grid : fields;
+  p.F = own ? fields : fields_in;
   p.pre = pre;
   hipStream_t st = (hipStream_t)stream;
   const int64_t n_clear = (int64_t)adjoint_slab_words(*t, pl);
@@ -484,9 +513,11 @@ int sr_far_adjoint_batch(const sr_tracks *t, const sr_grid *g, const sr_opts *o,
   if ((rc = step_prep(*t, SR_MODE_FAR, pre, st))) return rc;
   for (int64_t pb = 0; pb < t->n_particles; pb += pl.chunk) {
     const int64_t pe = std::min<int64_t>(t->n_particles, pb + pl.chunk);
-    const int64_t n_f = 6 * (pe - pb) * p.n_grid;
-    hipLaunchKernelGGL(k_adjoint_clear, dim3((unsigned)((n_f + 255) / 256)), dim3(256), 0, st, fields, n_f);
-    if ((rc = spectrum_fields(*t, *g, o->precision, pb, pe, pre, fields, st))) return rc;
+    if (own) {
+      const int64_t n_f = 6 * (pe - pb) * p.n_grid;
+      hipLaunchKernelGGL(k_adjoint_clear, dim3((unsigned)((n_f + 255) / 256)), dim3(256), 0, st, fields, n_f);
+      if ((rc = spectrum_fields(*t, *g, o->precision, pb, pe, pre, fields, st))) return rc;
+    }
     p.chunk_base = pb;
     p.chunk_n = pe - pb;
     const int64_t blocks = p.chunk_n * (int64_t)pl.groups;
@@ -502,6 +533,85 @@ int sr_far_adjoint_batch(const sr_tracks *t, const sr_grid *g, const sr_opts *o,
   const hipError_t e = hipGetLastError();
   if (e != hipSuccess) return fail(SR_ERR_HIP, "adjoint launch: %s", hipGetErrorString(e));
   return SR_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int sr_far_adjoint_workspace_bytes(const sr_tracks *tracks, const sr_grid *grid, const sr_opts *opts,
+                                   size_t *bytes) {
+  if (!bytes) return fail(SR_ERR_INVALID, "bytes is NULL");
+  *bytes = 0;
+  int rc = adjoint_check(tracks, grid, opts);
+  if (rc) return rc;
+  if (tracks->n_particles == 0) return SR_OK;
+  *bytes = adjoint_ws(*tracks, *grid, adjoint_plan(*tracks, *grid, *opts, false), true);
+  return SR_OK;
+}
+
+int sr_far_adjoint_batch(const sr_tracks *t, const sr_grid *g, const sr_opts *o, const double *const *dS,
+                         double *gx, double *gy, double *gz, double *gux, double *guy, double *guz,
+                         double *gw, void *workspace, size_t workspace_bytes, void *stream) {
+  return adjoint_run(t, g, o, dS, nullptr, gx, gy, gz, gux, guy, guz, gw, workspace, workspace_bytes, stream);
+}
+
+int sr_far_fields_workspace_bytes(const sr_tracks *tracks, const sr_grid *grid, const sr_opts *opts,
+                                  size_t *bytes) {
+  if (!bytes) return fail(SR_ERR_INVALID, "bytes is NULL");
+  *bytes = 0;
+  int rc = adjoint_check(tracks, grid, opts);
+  if (rc) return rc;
+  *bytes = 6 * (size_t)tracks->n_points * sizeof(double);
+  return SR_OK;
+}
+
+int sr_far_fields_batch(const sr_tracks *t, const sr_grid *g, const sr_opts *o, double *fields,
+                        double *const *outs, void *workspace, size_t workspace_bytes, void *stream) {
+  int rc = adjoint_check(t, g, o);
+  if (rc) return rc;
+  if (t->n_particles == 0) return SR_OK;
+  if (!fields) return fail(SR_ERR_INVALID, "fields is NULL");
+  const int n_out = (o->comp == SR_COMP_TOTAL) ? 1 : 3;
+  if (outs)
+    for (int c = 0; c < n_out; c++)
+      if (!outs[c]) return fail(SR_ERR_INVALID, "outs[%d] is NULL", c);
+  const size_t need = 6 * (size_t)t->n_points * sizeof(double);
+  if (!workspace || workspace_bytes < need)
+    return fail(SR_ERR_WORKSPACE, "workspace of %zu bytes needed, %zu given", need, workspace_bytes);
+  hipStream_t st = (hipStream_t)stream;
+  double *pre = (double *)workspace;
+  const int64_t n_grid = (int64_t)g->n_omega * g->n_theta * g->n_phi;
+  const int64_t n_f = 6 * t->n_particles * n_grid;
+  hipLaunchKernelGGL(k_adjoint_clear, dim3((unsigned)((n_f + 255) / 256)), dim3(256), 0, st, fields, n_f);
+  if ((rc = step_prep(*t, SR_MODE_FAR, pre, st))) return rc;
+  if ((rc = spectrum_fields(*t, *g, o->precision, 0, t->n_particles, pre, fields, st))) return rc;
+  if (outs)
+    hipLaunchKernelGGL(k_fields_spectrum, dim3((unsigned)((n_grid + 255) / 256)), dim3(256), 0, st, t->offsets,
+                       t->weight, t->dt, t->n_particles, n_grid, o->comp, (const double *)fields, outs[0],
+                       n_out > 1 ? outs[1] : nullptr, n_out > 1 ? outs[2] : nullptr);
+  const hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return fail(SR_ERR_HIP, "fields launch: %s", hipGetErrorString(e));
+  return SR_OK;
+}
+
+int sr_far_adjoint_fields_workspace_bytes(const sr_tracks *tracks, const sr_grid *grid, const sr_opts *opts,
+                                          size_t *bytes) {
+  if (!bytes) return fail(SR_ERR_INVALID, "bytes is NULL");
+  *bytes = 0;
+  int rc = adjoint_check(tracks, grid, opts);
+  if (rc) return rc;
+  if (tracks->n_particles == 0) return SR_OK;
+  *bytes = adjoint_ws(*tracks, *grid, adjoint_plan(*tracks, *grid, *opts, true), false);
+  return SR_OK;
+}
+
+int sr_far_adjoint_from_fields(const sr_tracks *t, const sr_grid *g, const sr_opts *o, const double *const *dS,
+                               const double *fields, double *gx, double *gy, double *gz, double *gux,
+                               double *guy, double *guz, double *gw, void *workspace, size_t workspace_bytes,
+                               void *stream) {
+  if (t && t->n_particles > 0 && !fields) return fail(SR_ERR_INVALID, "fields is NULL");
+  return adjoint_run(t, g, o, dS, fields, gx, gy, gz, gux, guy, guz, gw, workspace, workspace_bytes, stream);
 }
 
 }  // extern "C"

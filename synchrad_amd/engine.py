@@ -443,12 +443,67 @@ class SpectrumGraph:
         return self.outs
 
 
+def _adjoint_opts(comp, precision, tuning):
+    tun = tuning or Tuning()
+    opts = _abi.SrOpts()
+    opts.comp = _abi.COMPS[comp]
+    opts.precision = PRECISIONS[precision] if isinstance(precision, str) else int(precision)
+    opts.waves_per_block = tun.waves_per_block
+    opts.slices = tun.slices
+    return opts
+
+
+def field_bytes(batch: DeviceBatch, grid: DeviceGrid) -> int:
+    """Bytes of every particle's complex far field (6 doubles per particle and
+    spectral point), as :func:`far_fields` stores them."""
+    return 48 * batch.host.n_particles * grid.n_grid
+
+
+def far_fields(batch: DeviceBatch, grid: DeviceGrid, comp: str = "total", precision="double",
+               outs: Optional[List[torch.Tensor]] = None, stream=None):
+    """Every particle's complex far field (float64 [6, n_particles, n_grid]:
+    x, y, z as re/im pairs) and the spectra of ``comp`` ('total' or
+    'cartesian', one snapshot) formed from them.  Returns ``(fields, outs)``;
+    ``outs`` are added to when given.  The backward of a spectrum passes the
+    fields to :func:`far_adjoint` instead of recomputing them."""
+    L = _abi.load()
+    if grid.mode != "far":
+        raise ValueError("fields cover the far field")
+    n_out = {"total": 1, "cartesian": 3}.get(comp)
+    if n_out is None:
+        raise ValueError("fields cover comp 'total' and 'cartesian'")
+    dev = grid.device
+    shape = (1, grid.n_phi, grid.n1, grid.n_omega)
+    if outs is None:
+        outs = [torch.zeros(shape, dtype=torch.float64, device=dev) for _ in range(n_out)]
+    elif len(outs) != n_out or any(tuple(o.shape) != shape or o.dtype != torch.float64 or not o.is_contiguous()
+                                   for o in outs):
+        raise ValueError(f"{comp} needs {n_out} contiguous float64 outputs of shape {shape}")
+    fields = torch.empty((6, batch.host.n_particles, grid.n_grid), dtype=torch.float64, device=dev)
+    opts = _adjoint_opts(comp, precision, None)
+    tr, gr = batch.abi(), grid.abi()
+    nbytes = ctypes.c_size_t(0)
+    _abi.check(L.sr_far_fields_workspace_bytes(ctypes.byref(tr), ctypes.byref(gr), ctypes.byref(opts),
+                                               ctypes.byref(nbytes)))
+    ws = torch.empty(max(int(nbytes.value), 1), dtype=torch.uint8, device=dev)
+    optr = (ctypes.c_void_p * n_out)(*[o.data_ptr() for o in outs])
+    st = stream if stream is not None else torch.cuda.current_stream(dev)
+    _abi.check(L.sr_far_fields_batch(ctypes.byref(tr), ctypes.byref(gr), ctypes.byref(opts),
+                                     ctypes.c_void_p(fields.data_ptr()), optr, ctypes.c_void_p(ws.data_ptr()),
+                                     nbytes.value, ctypes.c_void_p(st.cuda_stream)))
+    if st != torch.cuda.current_stream(dev):
+        ws.record_stream(st)
+    return fields, outs
+
+
 def far_adjoint(batch: DeviceBatch, grid: DeviceGrid, dS: List[torch.Tensor], comp: str = "total",
-                precision="double", tuning: Optional[Tuning] = None, grads=None, stream=None):
+                precision="double", tuning: Optional[Tuning] = None, grads=None, stream=None,
+                fields: Optional[torch.Tensor] = None):
     """dL/d(tracks) of the far-field spectrum given ``dS`` = dL/dS (device
     float64, one array for 'total', three for 'cartesian', each shaped like the
     spectrum: (1, nPhi, nTheta, nOmega)).  Returns ``(gx, gy, gz, gux, guy, guz,
-    gw)`` device tensors (added to ``grads`` if given)."""
+    gw)`` device tensors (added to ``grads`` if given).  ``fields`` from
+    :func:`far_fields` (same batch, grid and precision) skips recomputing them."""
     L = _abi.load()
     if grid.mode != "far":
         raise ValueError("the adjoint covers the far field")
@@ -464,25 +519,27 @@ def far_adjoint(batch: DeviceBatch, grid: DeviceGrid, dS: List[torch.Tensor], co
             raise ValueError(f"dS arrays must have shape {shape}")
     dev = grid.device
     npts, npart = batch.host.n_points, batch.host.n_particles
+    if fields is not None and (tuple(fields.shape) != (6, npart, grid.n_grid) or fields.dtype != torch.float64
+                               or not fields.is_contiguous()):
+        raise ValueError(f"fields must be contiguous float64 of shape {(6, npart, grid.n_grid)}")
     if grads is None:
         grads = [torch.zeros(npts, dtype=torch.float64, device=dev) for _ in range(6)] + \
                 [torch.zeros(npart, dtype=torch.float64, device=dev)]
-    tun = tuning or Tuning()
-    opts = _abi.SrOpts()
-    opts.comp = _abi.COMPS[comp]
-    opts.precision = PRECISIONS[precision] if isinstance(precision, str) else int(precision)
-    opts.waves_per_block = tun.waves_per_block
-    opts.slices = tun.slices
+    opts = _adjoint_opts(comp, precision, tuning)
     tr, gr = batch.abi(), grid.abi()
     nbytes = ctypes.c_size_t(0)
-    _abi.check(L.sr_far_adjoint_workspace_bytes(ctypes.byref(tr), ctypes.byref(gr), ctypes.byref(opts),
-                                                ctypes.byref(nbytes)))
+    size_fn = L.sr_far_adjoint_workspace_bytes if fields is None else L.sr_far_adjoint_fields_workspace_bytes
+    _abi.check(size_fn(ctypes.byref(tr), ctypes.byref(gr), ctypes.byref(opts), ctypes.byref(nbytes)))
     ws = torch.empty(max(int(nbytes.value), 1), dtype=torch.uint8, device=dev)
     dptr = (ctypes.c_void_p * n_in)(*[d.data_ptr() for d in dS])
     st = stream if stream is not None else torch.cuda.current_stream(dev)
-    _abi.check(L.sr_far_adjoint_batch(ctypes.byref(tr), ctypes.byref(gr), ctypes.byref(opts), dptr,
-                                      *[g.data_ptr() for g in grads], ctypes.c_void_p(ws.data_ptr()),
-                                      nbytes.value, ctypes.c_void_p(st.cuda_stream)))
+    tail = [*[g.data_ptr() for g in grads], ctypes.c_void_p(ws.data_ptr()), nbytes.value,
+            ctypes.c_void_p(st.cuda_stream)]
+    if fields is None:
+        _abi.check(L.sr_far_adjoint_batch(ctypes.byref(tr), ctypes.byref(gr), ctypes.byref(opts), dptr, *tail))
+    else:
+        _abi.check(L.sr_far_adjoint_from_fields(ctypes.byref(tr), ctypes.byref(gr), ctypes.byref(opts), dptr,
+                                                ctypes.c_void_p(fields.data_ptr()), *tail))
     if st != torch.cuda.current_stream(dev):
         ws.record_stream(st)
     return tuple(grads)

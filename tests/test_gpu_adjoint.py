@@ -374,3 +374,81 @@ def test_adjoint_chunks_of_particles(precision, tol):
         r = rel_l2(full[c], np.concatenate(one[c]))
         print(f"chunked adjoint {precision} grad[{c}] vs one particle at a time: rel-L2 {r:.3e}", flush=True)
         assert r < tol
+
+
+@pytest.mark.parametrize("comp", ["total", "cartesian"])
+@pytest.mark.parametrize("precision", ["double", "float"])
+def test_fields_forward_equals_spectrum(comp, precision):
+    """The spectra formed from the kept per-particle fields equal the spectrum
+    kernel's (w dt^2 |F|^2 with the record's expressions, in particle order),
+    and each particle's stored field gives its own spectrum."""
+    wl = workloads.config2(n_particles=5, n_steps=1500, grid=(96, 8, 6))
+    ax = wl.axes
+    grid = engine.DeviceGrid("far", ax["omega"], ax["theta"], ax["phi"])
+    batch = engine.DeviceBatch(wl.batch)
+    ref = engine.spectrum(batch, grid, comp=comp, precision=precision, tuning=engine.Tuning(slices=1))
+    F, outs = engine.far_fields(batch, grid, comp=comp, precision=precision)
+    torch.cuda.synchronize()
+    for a, b in zip(outs, ref):
+        r = float(torch.linalg.norm(a - b) / torch.linalg.norm(b))
+        print(f"fields-derived {comp} {precision}: rel-L2 {r:.3e}", flush=True)
+        assert r < 1e-14
+    # particle 2 alone
+    t2 = wl.track_list()[2]
+    one = engine.spectrum(engine.DeviceBatch(engine.pack_tracks([t2], dt=wl.batch.dt)), grid, comp=comp,
+                          precision=precision)
+    f = F[:, 2, :].reshape(3, 2, -1)
+    sq = (f ** 2).sum(dim=1) * (wl.batch.weight[2] * wl.batch.dt ** 2)
+    got = [sq.sum(dim=0)] if comp == "total" else [sq[c] for c in range(3)]
+    for a, b in zip(got, one):
+        assert float(torch.linalg.norm(a - b.reshape(-1)) / torch.linalg.norm(b)) < 1e-13
+
+
+@pytest.mark.parametrize("precision,tol", [("double", 1e-11), ("float", 1e-6)])
+def test_adjoint_from_kept_fields_equals_recomputed(precision, tol):
+    """The backward given the forward's fields equals the one that recomputes
+    them, here over two chunks of the recomputing path's field workspace
+    (30 particles x 512 x 64 x 64).  The two paths split the directions into
+    different numbers of groups (one chunk of 30 particles vs chunks of ~20),
+    so the FP64 sums are regrouped; the momentum gradients, whose terms cancel,
+    show that as ~1e-13."""
+    wl = workloads.config2(n_particles=30, n_steps=200)
+    ax = wl.axes
+    grid = engine.DeviceGrid("far", ax["omega"], ax["theta"], ax["phi"])
+    batch = engine.DeviceBatch(wl.batch)
+    rng = np.random.default_rng(11)
+    G = torch.from_numpy(rng.standard_normal((1, ax["phi"].size, ax["theta"].size, ax["omega"].size))).cuda()
+    F, _ = engine.far_fields(batch, grid, comp="total", precision=precision)
+    a = engine.far_adjoint(batch, grid, [G], precision=precision, fields=F)
+    b = engine.far_adjoint(batch, grid, [G], precision=precision)
+    torch.cuda.synchronize()
+    for c in range(7):
+        r = float(torch.linalg.norm(a[c] - b[c]) / torch.linalg.norm(b[c]))
+        print(f"kept-fields adjoint {precision} grad[{c}]: rel-L2 {r:.3e}", flush=True)
+        assert r < tol
+
+
+def test_autograd_with_and_without_kept_fields(monkeypatch):
+    """loss.backward() gives the same gradients whether the forward keeps the
+    fields or the backward recomputes them."""
+    wl = workloads.config2(n_particles=4, n_steps=1200, grid=(64, 6, 4))
+    ax = wl.axes
+    grid = engine.DeviceGrid("far", ax["omega"], ax["theta"], ax["phi"])
+    hb = wl.batch
+    rng = np.random.default_rng(3)
+    W = torch.from_numpy(rng.random((1, ax["phi"].size, ax["theta"].size, ax["omega"].size))).cuda()
+
+    def grads():
+        leaves = [torch.tensor(getattr(hb, c), device="cuda", requires_grad=True)
+                  for c in ("x", "y", "z", "ux", "uy", "uz", "weight")]
+        S = autograd.far_spectrum(*leaves, hb, grid, comp="total", precision="double")[0]
+        (W * S ** 2).sum().backward()
+        return S.detach(), [t.grad for t in leaves]
+
+    S_keep, g_keep = grads()
+    monkeypatch.setattr(autograd, "KEEP_FIELDS_BYTES", 0)
+    S_re, g_re = grads()
+    torch.cuda.synchronize()
+    assert float(torch.linalg.norm(S_keep - S_re) / torch.linalg.norm(S_re)) < 1e-14
+    for a, b in zip(g_keep, g_re):
+        assert float(torch.linalg.norm(a - b) / torch.linalg.norm(b)) < 1e-12

@@ -179,3 +179,18 @@ def test_config3_lengths_are_log_uniform():
     n = np.diff(wl.batch.offsets)
     assert n.min() >= 200 and n.max() <= 20000
     assert abs(np.mean(np.log(n)) - 0.5 * (np.log(200) + np.log(20000))) < 0.25
+
+
+def test_bench_refuses_more_than_one_gpu():
+    """bench.py runs on one GPU; --gpus N > 1 exits with a message before
+    touching a device."""
+    import json
+    import os
+    import subprocess
+    import sys
+    repo = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    r = subprocess.run([sys.executable, os.path.join(repo, "bench.py"), "--gpus", "2"],
+                       capture_output=True, text=True, timeout=120)
+    assert r.returncode == 2
+    msg = json.loads(r.stdout.strip().splitlines()[-1])
+    assert "one GPU" in msg["error"]
