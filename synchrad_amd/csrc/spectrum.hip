@@ -163,6 +163,32 @@ constexpr size_t acc_lds_bytes(int T) {
   return (size_t)M * T * sizeof(double);
 }
 
+// amplitude words per staged step: far split A.e_theta, A.e_phi; far FP64
+// A_x, A_y, A_z; near v1 and v2 (three components each)
+template <int MODE, int PREC>
+constexpr int amp_words() {
+  return MODE == SR_MODE_FAR ? (PREC == SR_PREC_SPLIT ? 2 : 3) : 6;
+}
+
+// Dynamic LDS before the Kahan state: one tile of T staged steps -- tau
+// (double), the split threshold (double), the recurrence rotation (2 floats)
+// and the amplitude words -- sized by the launched block, so that one-wave
+// blocks do not hold a 256-step tile's LDS.
+template <int MODE, int PREC, int SWEEP>
+constexpr size_t stage_lds_bytes(int T) {
+  constexpr bool split = PREC == SR_PREC_SPLIT;
+  return (size_t)T * (sizeof(double) + (split ? sizeof(double) : 0) + (SWEEP == SWEEP_REC ? 2 * sizeof(float) : 0) +
+                      amp_words<MODE, PREC>() * (split ? sizeof(float) : sizeof(double)));
+}
+
+// rows of a [n][T] LDS array: rows[c][j]
+template <typename X>
+struct LdsRows {
+  X *base;
+  int T;
+  __device__ X *operator[](int c) const { return base + c * T; }
+};
+
 // Register budget per variant, as the minimum number of waves each SIMD must
 // be able to hold: the split sweeps keep their Kahan state in LDS, so their
 // registers hold the partial sums of M omegas plus one staged step group.
@@ -229,17 +255,13 @@ k_spectrum(const KParams p) {
   // (n.A = c1 (1 - n.bbar) - c2 n.a = 0 with c1 = (n.a) c2^2), so it is kept
   // as its two components on e_theta, e_phi.  Far FP64: A_x, A_y, A_z as the
   // reference sums them.  Near: v1 = (b - n)/R and v2 = n/R^2.
-  constexpr int NA = (MODE == SR_MODE_FAR) ? (kSplit ? 2 : 3) : 6;
+  constexpr int NA = amp_words<MODE, PREC>();
   constexpr int NC = 2 * NA;                           // real accumulators per omega
   constexpr bool kCompact = kSplit && kRec && (SR_COMPACT != 0);
   constexpr int kMaxW = kMaxT / kWave;                 // waves per block
   constexpr int kMaxRuns = kMaxW * 4;                  // 64-omega runs per block (M <= 4)
 
-  __shared__ __align__(16) double s_tau[kMaxT];
-  __shared__ __align__(16) Amp s_amp[NA][kMaxT];
-  __shared__ __align__(16) double s_thr[kSplit ? kMaxT : 1];
-  __shared__ __align__(16) float s_rot[2][kRec ? kMaxT : 1];
-  extern __shared__ __align__(16) float s_dyn[];
+  extern __shared__ __align__(16) char s_dyn_raw[];
   // compaction: smallest |omega|/2pi of each run, its rank, the per-wave
   // bucket histogram, its scan, and the step count of each rank
   __shared__ double s_wmin[kCompact ? kMaxRuns : 1];
@@ -253,6 +275,13 @@ k_spectrum(const KParams p) {
   const int W = T / kWave;
   const int wv = tid / kWave;
   const int lane = tid % kWave;
+  // dynamic LDS: the staged tile (s_tau[T], s_thr[T], s_rot[2][T],
+  // s_amp[NA][T]), then the Kahan state, then the particle sums
+  double *const s_tau = (double *)s_dyn_raw;
+  double *const s_thr = s_tau + T;
+  const LdsRows<float> s_rot{(float *)(s_tau + (kSplit ? 2 : 1) * T), T};
+  const LdsRows<Amp> s_amp{(Amp *)((char *)s_rot.base + (kRec ? 2 * T * sizeof(float) : 0)), T};
+  float *const s_dyn = (float *)(s_dyn_raw + stage_lds_bytes<MODE, PREC, SWEEP>(T));
   float *const s_ks = s_dyn + tid;                       // S[c*M+k] at s_ks[(c*M+k)*T]
   float *const s_kc = s_dyn + (size_t)NC * M * T + tid;  // C likewise
   // particle sums of the single output, k-th omega at s_acc[k*T]
@@ -969,7 +998,8 @@ Plan make_plan(const sr_grid &g, const sr_opts &o, int64_t n_particles, uint32_t
 
 template <int MODE, int PREC, int SWEEP, int M>
 hipError_t launch_t(const KParams &kp, int64_t grid, int threads, hipStream_t st) {
-  const size_t lds = ((PREC == SR_PREC_SPLIT) ? kahan_lds_bytes<MODE, M>(threads) : 0) +
+  const size_t lds = stage_lds_bytes<MODE, PREC, SWEEP>(threads) +
+                     ((PREC == SR_PREC_SPLIT) ? kahan_lds_bytes<MODE, M>(threads) : 0) +
                      (kp.acc_lds ? acc_lds_bytes<M>(threads) : 0);
   hipLaunchKernelGGL((k_spectrum<MODE, PREC, SWEEP, M>), dim3((unsigned)grid), dim3(threads), lds, st,
                      kp);
