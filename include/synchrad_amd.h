@@ -23,7 +23,7 @@
 extern "C" {
 #endif
 
-#define SR_ABI_VERSION 1
+#define SR_ABI_VERSION 2
 
 typedef enum sr_status {
   SR_OK = 0,
@@ -126,6 +126,9 @@ typedef struct sr_opts {
   int32_t persistent;       /* 0 = one block per tile; >0 = that many blocks loop over tiles */
   int32_t sweep;            /* SR_SWEEP_* */
   const double *form_factor;/* device [n_omega]; far cartesian_complex only */
+  int32_t tile_order;       /* 0 = batch calls launch the far-field tiles heaviest first (estimated
+                               from sampled steps) when the launch is large; 1 = index order.
+                               Results are identical either way. */
 } sr_opts;
 
 int sr_abi_version(void);

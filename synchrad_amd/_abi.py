@@ -10,7 +10,7 @@ import os
 
 from ._build import LIB
 
-SR_ABI_VERSION = 1
+SR_ABI_VERSION = 2
 
 SR_OK = 0
 SR_ERR_INVALID = -1
@@ -74,7 +74,7 @@ class SrOpts(ctypes.Structure):
     _fields_ = [
         ("comp", _i32), ("precision", _i32), ("omega_per_lane", _i32),
         ("waves_per_block", _i32), ("slices", _i32), ("persistent", _i32),
-        ("sweep", _i32), ("form_factor", _vp),
+        ("sweep", _i32), ("form_factor", _vp), ("tile_order", _i32),
     ]
 
 

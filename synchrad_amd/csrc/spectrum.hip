@@ -136,6 +136,8 @@ struct KParams {
   int32_t slices;
   int64_t n_tiles;
   int64_t slice_lo[kMaxSlices + 1];
+  // launch order of the tiles (heaviest first), or nullptr: index order
+  const int32_t *tile_order;
 };
 
 // ---------------------------------------------------------------------------
@@ -291,7 +293,8 @@ k_spectrum(const KParams p) {
   const bool spheric = (p.comp == SR_COMP_SPHERIC || p.comp == SR_COMP_SPHERIC_COMPLEX);
   const double rot_rev = (double)T * p.omega_step * kInv2Pi;   // revolutions of the k -> k+1 step
 
-  for (int64_t tile = blockIdx.x; tile < p.n_tiles; tile += gridDim.x) {
+  for (int64_t ti = blockIdx.x; ti < p.n_tiles; ti += gridDim.x) {
+    const int64_t tile = p.tile_order ? (int64_t)p.tile_order[ti] : ti;
     const uint32_t chunk = (uint32_t)(tile % p.n_chunks);
     const int64_t rest = tile / p.n_chunks;
     const uint32_t dir = (uint32_t)(rest % p.n_dir);
@@ -869,6 +872,75 @@ k_spectrum(const KParams p) {
 }
 
 
+// ---------------------------------------------------------------------------
+// heaviest tiles first
+// ---------------------------------------------------------------------------
+// A far-field tile's time grows with the share of its (omega, step) pairs that
+// the filter keeps, which varies several-fold over the directions (near the
+// beam axis most pairs are kept).  Launched in index order, the last blocks
+// keep a few CUs busy while the rest of the GPU idles; so each tile's time is
+// estimated from a sample of steps and the tiles are launched heaviest first.
+// Every tile writes only its own outputs, so the order changes no result.
+constexpr int kOrderSamples = 256;
+constexpr int64_t kOrderMinTiles = 2048;
+constexpr int64_t kOrderMaxTiles = 65536;
+constexpr double kOrderMinEvals = 1e10;
+constexpr float kOrderStepCost = 0.1f;   // a step's fixed work, in units of a fully kept step's sweep
+
+// cost[tile] for every (direction, omega chunk), the same for each particle
+// slice: sum over sampled intervals of kOrderStepCost + the share of the
+// chunk's |omega|/2pi range below the interval's threshold 1/(2|dtau|)
+__global__ void k_tile_cost(const KParams p, int64_t run_len, float *cost) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= (int64_t)p.n_dir * p.n_chunks) return;
+  const uint32_t chunk = (uint32_t)(i % p.n_chunks);
+  const uint32_t dir = (uint32_t)(i / p.n_chunks);
+  const uint32_t iPhi = dir / p.n_theta;
+  const uint32_t iTh = dir - iPhi * p.n_theta;
+  const double st = p.sin_t[iTh], ct = p.cos_t[iTh];
+  const double nx = st * p.cos_p[iPhi], ny = st * p.sin_p[iPhi], nz = ct;
+  const int64_t first = (int64_t)chunk * run_len;
+  const int64_t last = min(first + run_len, (int64_t)p.n_omega);
+  double wlo = 1e300, whi = 0.0;
+  for (int64_t w = first; w < last; w++) {
+    const double a = fabs(p.omega[w]) * kInv2Pi;
+    wlo = fmin(wlo, a);
+    whi = fmax(whi, a);
+  }
+  float c = 0.f;
+  const int64_t n = p.n_points;
+  for (int k = 0; k < kOrderSamples; k++) {
+    const int64_t g = 1 + (int64_t)k * (n - 1) / kOrderSamples;   // intervals (g-1, g); rarely across particles
+    const double dtau = p.dt - (nx * (p.x[g] - p.x[g - 1]) + ny * (p.y[g] - p.y[g - 1]) + nz * (p.z[g] - p.z[g - 1]));
+    const double thr = 0.5 / fabs(dtau);
+    double f = (whi > wlo) ? (thr - wlo) / (whi - wlo) : (thr > wlo ? 1.0 : 0.0);
+    f = fmin(fmax(f, 0.0), 1.0);
+    c += kOrderStepCost + (float)f;
+  }
+  if (!(c >= 0.f && c <= 1e30f)) c = 0.f;   // a strict order below needs no NaN
+  for (int q = 0; q < p.slices; q++) cost[chunk + (int64_t)p.n_chunks * (dir + (int64_t)p.n_dir * q)] = c;
+}
+
+// order[rank] = tile, rank = tiles that cost more, or as much with a smaller
+// index: a permutation (the key (-cost, index) is a strict total order)
+__global__ void __launch_bounds__(256) k_tile_rank(const float *cost, int64_t n, int32_t *order) {
+  __shared__ float s_c[256];
+  const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  const float ci = i < n ? cost[i] : 0.f;
+  int64_t rank = 0;
+  for (int64_t j0 = 0; j0 < n; j0 += 256) {
+    __syncthreads();
+    if (j0 + threadIdx.x < n) s_c[threadIdx.x] = cost[j0 + threadIdx.x];
+    __syncthreads();
+    const int m = (int)min((int64_t)256, n - j0);
+    for (int jj = 0; jj < m; jj++) {
+      const float cj = s_c[jj];
+      rank += (cj > ci || (cj == ci && j0 + jj < i)) ? 1 : 0;
+    }
+  }
+  if (i < n) order[rank] = (int32_t)i;
+}
+
 // Direction-independent data of every sample (the stage's inputs that do not
 // depend on the spectral point), computed once per launch: far a, bbar of the
 // interval (i, i+1) (kernel_farfield.cl:74-83); near b_i (kernel_nearfield.cl:
@@ -1056,6 +1128,7 @@ int check_opts(const sr_grid *g, const sr_opts *o) {
   if (o->slices < 0 || o->slices > kMaxSlices)
     return fail(SR_ERR_INVALID, "slices must be in [0, %d]", kMaxSlices);
   if (o->persistent < 0) return fail(SR_ERR_INVALID, "persistent must be >= 0");
+  if (o->tile_order != 0 && o->tile_order != 1) return fail(SR_ERR_INVALID, "tile_order must be 0 or 1");
   if (o->sweep < SR_SWEEP_AUTO || o->sweep > SR_SWEEP_RECURRENCE)
     return fail(SR_ERR_INVALID, "sweep must be SR_SWEEP_AUTO, _DIRECT or _RECURRENCE");
   if (o->sweep == SR_SWEEP_RECURRENCE && (o->precision != SR_PREC_SPLIT || g->omega_step == 0.0))
@@ -1110,6 +1183,15 @@ size_t scratch_bytes(const Plan &pl, int n_out, int64_t n_grid, uint32_t n_snaps
 size_t pre_bytes(int mode, int64_t n_points) {
   return (size_t)(mode == SR_MODE_FAR ? 6 : 3) * (size_t)n_points * sizeof(double);
 }
+// the tile costs and launch order (batch calls in the far field with enough
+// tiles and work to have a tail worth ordering), 0 otherwise
+size_t order_bytes(const Plan &pl, const sr_grid &g, const sr_opts &o, int64_t n_points) {
+  const double evals = (double)n_points * g.n_omega * g.n_theta * g.n_phi;
+  if (g.mode != SR_MODE_FAR || o.tile_order != 0 || n_points < 2 || pl.blocks < kOrderMinTiles ||
+      pl.blocks > kOrderMaxTiles || evals < kOrderMinEvals)
+    return 0;
+  return ((size_t)pl.blocks * (sizeof(float) + sizeof(int32_t)) + 255) / 256 * 256;
+}
 
 // want_pre: compute the step data into the workspace (batch calls); the
 // single-particle entry points have no workspace and compute it in the stage
@@ -1124,9 +1206,23 @@ int run(KParams &kp, const Plan &pl, const sr_grid &g, const sr_opts &o, void *w
   // per-particle outputs are disjoint: slices write them in place
   const size_t need = kp.part_stride ? 0 : scratch_bytes(pl, kp.n_out, kp.n_grid, kp.n_snaps);
   const size_t need_pre = want_pre ? pre_bytes(g.mode, kp.n_points) : 0;
+  const size_t need_order = want_pre ? order_bytes(pl, g, o, kp.n_points) : 0;
   hipError_t e;
-  if (need + need_pre && (!ws || ws_bytes < need + need_pre))
-    return fail(SR_ERR_WORKSPACE, "workspace of %zu bytes needed, %zu given", need + need_pre, ws_bytes);
+  if (need + need_pre + need_order && (!ws || ws_bytes < need + need_pre + need_order))
+    return fail(SR_ERR_WORKSPACE, "workspace of %zu bytes needed, %zu given", need + need_pre + need_order,
+                ws_bytes);
+  kp.tile_order = nullptr;
+  if (need_order) {
+    float *cost = (float *)((char *)ws + need + need_pre);
+    int32_t *order = (int32_t *)(cost + pl.blocks);
+    const int64_t pairs = (int64_t)kp.n_dir * kp.n_chunks;
+    hipLaunchKernelGGL(k_tile_cost, dim3((unsigned)((pairs + 255) / 256)), dim3(256), 0, st, kp, run, cost);
+    hipLaunchKernelGGL(k_tile_rank, dim3((unsigned)((pl.blocks + 255) / 256)), dim3(256), 0, st,
+                       (const float *)cost, (int64_t)pl.blocks, order);
+    e = hipGetLastError();
+    if (e != hipSuccess) return fail(SR_ERR_HIP, "tile order launch: %s", hipGetErrorString(e));
+    kp.tile_order = order;
+  }
   if (need_pre) {
     kp.pre = (const double *)((char *)ws + need);
     const int64_t n = kp.n_points;
@@ -1289,7 +1385,7 @@ int sr_workspace_bytes(const sr_tracks *tracks, const sr_grid *grid, const sr_op
   const Plan pl = make_plan(*grid, *opts, tracks->n_particles, tracks->n_snaps);
   *bytes = scratch_bytes(pl, outputs_for(grid->mode, opts->comp),
                          (int64_t)grid->n_omega * grid->n_theta * grid->n_phi, tracks->n_snaps) +
-           pre_bytes(grid->mode, tracks->n_points);
+           pre_bytes(grid->mode, tracks->n_points) + order_bytes(pl, *grid, *opts, tracks->n_points);
   return SR_OK;
 }
 

@@ -301,6 +301,7 @@ class Tuning:
     slices: int = 0
     persistent: int = 0
     sweep: str = "auto"
+    tile_order: int = 0    # 0: large far-field launches run their heaviest tiles first; 1: index order
 
 
 def spectrum(batch: DeviceBatch, grid: DeviceGrid, comp: str = "total", precision="double",
@@ -344,6 +345,7 @@ def spectrum(batch: DeviceBatch, grid: DeviceGrid, comp: str = "total", precisio
     opts.waves_per_block = tun.waves_per_block
     opts.slices = tun.slices
     opts.persistent = tun.persistent
+    opts.tile_order = tun.tile_order
     opts.sweep = SWEEPS[tun.sweep]
     ff = None
     if comp == "cartesian_complex" and grid.mode == "far":
@@ -388,6 +390,7 @@ def workspace_bytes(batch: DeviceBatch, grid: DeviceGrid, comp: str = "total", p
     opts.waves_per_block = tun.waves_per_block
     opts.slices = tun.slices
     opts.persistent = tun.persistent
+    opts.tile_order = tun.tile_order
     opts.sweep = SWEEPS[tun.sweep]
     opts.form_factor = 1 if (comp == "cartesian_complex" and grid.mode == "far") else None
     tr, gr = batch.abi(), grid.abi()

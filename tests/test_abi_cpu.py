@@ -108,6 +108,7 @@ def _err():
     (dict(tracks=dict(snaps_stride=5)), "snaps_stride"),
     (dict(tracks=dict(dt=0.0)), "dt"),
     (dict(tracks=dict(x=None)), "NULL"),
+    (dict(opts=dict(tile_order=2)), "tile_order"),
 ])
 def test_validation_errors(case, msg):
     g = _grid(**case.get("grid", {}))
@@ -136,9 +137,16 @@ def test_workspace_plan():
     # sample in the far field (a, bbar), 3 in the near field (b)
     t = _tracks(n_particles=2000)
     pre = 6 * 8 * t.n_points
-    # large grid, one snapshot: no particle slices, only the step data
+    # large grid, one snapshot: no particle slices; the step data, and a cost
+    # and a launch index per tile (4096 tiles, launched heaviest first)
     rc, n = _ws(t, _grid(), _opts())
+    assert rc == 0 and n == pre + 8 * 4096
+    # index order: only the step data
+    rc, n = _ws(t, _grid(), _opts(tile_order=1))
     assert rc == 0 and n == pre
+    # the near field is not reordered (no filter-dependent work per tile)
+    rc, n = _ws(t, _grid(mode=_abi.SR_MODE_NEAR, radius=FAKE, l_screen=1.0), _opts())
+    assert rc == 0 and n == 3 * 8 * t.n_points
     # small grid: particle slices add n_slices * n_out * n_grid doubles
     g = _grid(n_omega=128, n_theta=32, n_phi=32)
     t256 = _tracks(n_particles=256)

@@ -368,3 +368,25 @@ def test_reference_test_script_runs_unchanged():
         print(f"deviation from the analytic estimate: {abs(e - e_th) / e_th:.4f}", flush=True)
         assert abs(e - e_th) / e_th < 0.03
     assert abs(e32 - e64) / e64 < 1e-6
+
+
+@pytest.mark.parametrize("precision", ["float", "double"])
+@pytest.mark.parametrize("comp,slices", [("total", 0), ("cartesian", 2)])
+def test_heaviest_tiles_first_changes_no_result(precision, comp, slices):
+    """Large far-field launches run their tiles in an estimated heaviest-first
+    order; every tile writes only its own outputs, so the spectra equal the
+    index-order launch bit for bit (4096 directions x 1 omega chunk, 6 x 8000
+    steps: 1.3e10 evaluations, above the size where the order is used)."""
+    wl = workloads.config2(n_particles=6, n_steps=8000, grid=(64, 64, 64))
+    ax = wl.axes
+    grid = engine.DeviceGrid("far", ax["omega"], ax["theta"], ax["phi"])
+    batch = engine.DeviceBatch(wl.batch)
+    on = engine.Tuning(slices=slices)
+    off = engine.Tuning(slices=slices, tile_order=1)
+    assert engine.workspace_bytes(batch, grid, comp, precision, on) > \
+        engine.workspace_bytes(batch, grid, comp, precision, off)
+    a = engine.spectrum(batch, grid, comp=comp, precision=precision, tuning=on)
+    b = engine.spectrum(batch, grid, comp=comp, precision=precision, tuning=off)
+    torch.cuda.synchronize()
+    for x, y in zip(a, b):
+        assert torch.equal(x, y)
