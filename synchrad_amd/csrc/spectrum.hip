@@ -881,11 +881,17 @@ k_spectrum(const KParams p) {
 // keep a few CUs busy while the rest of the GPU idles; so each tile's time is
 // estimated from a sample of steps and the tiles are launched heaviest first.
 // Every tile writes only its own outputs, so the order changes no result.
-constexpr int kOrderSamples = 256;
+#ifndef SR_ORDER_SAMPLES
+#define SR_ORDER_SAMPLES 256
+#endif
+#ifndef SR_ORDER_STEP_COST
+#define SR_ORDER_STEP_COST 0.1f
+#endif
+constexpr int kOrderSamples = SR_ORDER_SAMPLES;
 constexpr int64_t kOrderMinTiles = 2048;
 constexpr int64_t kOrderMaxTiles = 65536;
 constexpr double kOrderMinEvals = 1e10;
-constexpr float kOrderStepCost = 0.1f;   // a step's fixed work, in units of a fully kept step's sweep
+constexpr float kOrderStepCost = SR_ORDER_STEP_COST;   // a step's fixed work, in units of a fully kept step's sweep
 
 // cost[tile] for every (direction, omega chunk), the same for each particle
 // slice: sum over sampled intervals of kOrderStepCost + the share of the
@@ -922,7 +928,9 @@ __global__ void k_tile_cost(const KParams p, int64_t run_len, float *cost) {
 }
 
 // order[rank] = tile, rank = tiles that cost more, or as much with a smaller
-// index: a permutation (the key (-cost, index) is a strict total order)
+// index: a permutation (the key (-cost, index) is a strict total order).
+// (Keeping the particle slices in launch order and sorting only within each,
+// for cache locality, measured 10 % slower on the 20000-particle batch.)
 __global__ void __launch_bounds__(256) k_tile_rank(const float *cost, int64_t n, int32_t *order) {
   __shared__ float s_c[256];
   const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
@@ -1038,9 +1046,12 @@ Plan make_plan(const sr_grid &g, const sr_opts &o, int64_t n_particles, uint32_t
       const int64_t chunks = (n_om + run - 1) / run;
       const double eff = (double)n_om / (double)(chunks * run);
       if (eff < 0.75) continue;
-      // M=2 with 4 waves measured fastest for both sweeps
+      // M=2 measured fastest for both sweeps; far-field recurrence blocks of 2
+      // waves (with the tiles launched heaviest first, the finer tiles balance
+      // better: +8 % over 4 waves on the betatron beam), otherwise 4 waves
       const double mpref = (m == 2) ? 0.02 : 0.0;
-      const double score = eff + 0.01 * w + mpref;
+      const double wpref = (pl.sweep == SWEEP_REC && g.mode == SR_MODE_FAR && w == 2) ? 0.03 : 0.0;
+      const double score = eff + 0.01 * w + mpref + wpref;
       if (score > best_score) { best_score = score; best_w = w; best_m = m; }
     }
   }
