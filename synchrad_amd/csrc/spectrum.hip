@@ -891,6 +891,10 @@ constexpr int kOrderSamples = SR_ORDER_SAMPLES;
 constexpr int64_t kOrderMinTiles = 2048;
 constexpr int64_t kOrderMaxTiles = 65536;
 constexpr double kOrderMinEvals = 1e10;
+// the ranking compares every pair of tiles (0.6 ms for 8192 tiles); it is
+// used when the launch has at least kOrderEvalsPerTile2 * tiles^2 evaluations
+// (about 100 times the ranking's time at 5e12 evaluations/s)
+constexpr double kOrderEvalsPerTile2 = 4.5e3;
 constexpr float kOrderStepCost = SR_ORDER_STEP_COST;   // a step's fixed work, in units of a fully kept step's sweep
 
 // cost[tile] for every (direction, omega chunk), the same for each particle
@@ -1199,7 +1203,8 @@ size_t pre_bytes(int mode, int64_t n_points) {
 size_t order_bytes(const Plan &pl, const sr_grid &g, const sr_opts &o, int64_t n_points) {
   const double evals = (double)n_points * g.n_omega * g.n_theta * g.n_phi;
   if (g.mode != SR_MODE_FAR || o.tile_order != 0 || n_points < 2 || pl.blocks < kOrderMinTiles ||
-      pl.blocks > kOrderMaxTiles || evals < kOrderMinEvals)
+      pl.blocks > kOrderMaxTiles || evals < kOrderMinEvals ||
+      evals < kOrderEvalsPerTile2 * (double)pl.blocks * (double)pl.blocks)
     return 0;
   return ((size_t)pl.blocks * (sizeof(float) + sizeof(int32_t)) + 255) / 256 * 256;
 }

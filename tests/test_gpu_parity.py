@@ -371,13 +371,14 @@ def test_reference_test_script_runs_unchanged():
 
 
 @pytest.mark.parametrize("precision", ["float", "double"])
-@pytest.mark.parametrize("comp,slices", [("total", 0), ("cartesian", 2)])
-def test_heaviest_tiles_first_changes_no_result(precision, comp, slices):
+@pytest.mark.parametrize("comp,slices,n_particles", [("total", 1, 20), ("cartesian", 2, 80)])
+def test_heaviest_tiles_first_changes_no_result(precision, comp, slices, n_particles):
     """Large far-field launches run their tiles in an estimated heaviest-first
     order; every tile writes only its own outputs, so the spectra equal the
-    index-order launch bit for bit (4096 directions x 1 omega chunk, 6 x 8000
-    steps: 1.3e10 evaluations, above the size where the order is used)."""
-    wl = workloads.config2(n_particles=6, n_steps=8000, grid=(64, 64, 64))
+    index-order launch bit for bit (2048 directions x 1 omega chunk x 1 or 2
+    particle slices, 20 or 80 x 8000 steps: 2.1e10 and 8.4e10 evaluations,
+    above the sizes where the order is used)."""
+    wl = workloads.config2(n_particles=n_particles, n_steps=8000, grid=(64, 32, 64))
     ax = wl.axes
     grid = engine.DeviceGrid("far", ax["omega"], ax["theta"], ax["phi"])
     batch = engine.DeviceBatch(wl.batch)
