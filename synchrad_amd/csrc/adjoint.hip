@@ -90,9 +90,15 @@ __global__ void __launch_bounds__(kMaxT) k_adjoint(const AParams p) {
   constexpr int NCW = TWO ? 2 : 3;   // coefficient words per omega: re 0..NCW-1, im NCW..2NCW-1
   constexpr int NW = 2 * NCW;
   __shared__ double s_cw[kAdjMaxOmega];     // omega (F64) or omega/2pi (split)
+  __shared__ float s_cwf[kSplit ? kAdjMaxOmega : 1];   // split: s_cw as the sums' FP32 factor
   __shared__ double s_red[kMaxT / kWave];
   extern __shared__ __align__(16) char s_dyn_raw[];
-  Acc *const s_cf = (Acc *)s_dyn_raw;       // [dir_batch][NW][n_omega]: H_c F_c
+  // H_c F_c of each direction of the batch, re words then im words: split
+  // [dir_batch][n_omega][NW] (one record per omega, 16 bytes with two words,
+  // read in one access); FP64 [dir_batch][NW][n_omega] (word-major keeps the
+  // FP64 sweep within 128 registers)
+  Acc *const s_cf = (Acc *)s_dyn_raw;
+
 
   const int T = blockDim.x;
   const int tid = threadIdx.x;
@@ -101,6 +107,9 @@ __global__ void __launch_bounds__(kMaxT) k_adjoint(const AParams p) {
   const int64_t ip = p.chunk_base + lp;
   const double dt = p.dt;
   const int n_om = (int)p.n_omega;
+  auto cf_at = [&](int d, int j, int k) -> size_t {
+    return kSplit ? ((size_t)d * n_om + j) * NW + k : ((size_t)d * NW + k) * n_om + j;
+  };
 
   const Particle pt = load_particle(p, ip);
   double gw_acc = 0.0;
@@ -152,22 +161,24 @@ __global__ void __launch_bounds__(kMaxT) k_adjoint(const AParams p) {
       }
 #pragma unroll
       for (int c = 0; c < 3; c++) gw_acc += dt * dt * G[c] * (Fr[c] * Fr[c] + Fi[c] * Fi[c]);
-      Acc *cf = s_cf + (size_t)d * NW * n_om;
       if constexpr (TWO) {
         const double H = 2.0 * wdt2 * G[0];
-        cf[0 * n_om + wi] = (Acc)(H * (Fr[0] * et[0] + Fr[1] * et[1] + Fr[2] * et[2]));
-        cf[1 * n_om + wi] = (Acc)(H * (Fr[0] * ep[0] + Fr[1] * ep[1]));
-        cf[2 * n_om + wi] = (Acc)(H * (Fi[0] * et[0] + Fi[1] * et[1] + Fi[2] * et[2]));
-        cf[3 * n_om + wi] = (Acc)(H * (Fi[0] * ep[0] + Fi[1] * ep[1]));
+        s_cf[cf_at(d, wi, 0)] = (Acc)(H * (Fr[0] * et[0] + Fr[1] * et[1] + Fr[2] * et[2]));
+        s_cf[cf_at(d, wi, 1)] = (Acc)(H * (Fr[0] * ep[0] + Fr[1] * ep[1]));
+        s_cf[cf_at(d, wi, 2)] = (Acc)(H * (Fi[0] * et[0] + Fi[1] * et[1] + Fi[2] * et[2]));
+        s_cf[cf_at(d, wi, 3)] = (Acc)(H * (Fi[0] * ep[0] + Fi[1] * ep[1]));
       } else {
 #pragma unroll
         for (int c = 0; c < 3; c++) {
           const double H = 2.0 * wdt2 * G[c];
-          cf[c * n_om + wi] = (Acc)(H * Fr[c]);
-          cf[(3 + c) * n_om + wi] = (Acc)(H * Fi[c]);
+          s_cf[cf_at(d, wi, c)] = (Acc)(H * Fr[c]);
+          s_cf[cf_at(d, wi, 3 + c)] = (Acc)(H * Fi[c]);
         }
       }
-      if (d == 0) s_cw[wi] = kSplit ? p.omega[wi] * kInv2Pi : p.omega[wi];
+      if (d == 0) {
+        s_cw[wi] = kSplit ? p.omega[wi] * kInv2Pi : p.omega[wi];
+        if (kSplit) s_cwf[wi] = (float)s_cw[wi];
+      }
     }
     __syncthreads();   // tables complete
     const bool rot = kSplit && p.domega_rev > 0.0 && s_cw[0] >= 0.0;
@@ -195,7 +206,7 @@ __global__ void __launch_bounds__(kMaxT) k_adjoint(const AParams p) {
       for (int d = 0; d < nb; d++) {
         double nv[3], et[3], ep[3];
         geom(d0 + (uint32_t)d, nv, et, ep);
-        const Acc *cf = s_cf + (size_t)d * NW * n_om;
+
         const double tau = time - (xs * nv[0] + ys * nv[1] + zs * nv[2]);
         const double tau_prev = (s > 0) ? tp - (xp * nv[0] + yp * nv[1] + zp * nv[2]) : 0.0;
         double c1 = a[0] * nv[0] + a[1] * nv[1] + a[2] * nv[2];
@@ -209,12 +220,22 @@ __global__ void __launch_bounds__(kMaxT) k_adjoint(const AParams p) {
 #pragma unroll
         for (int c = 0; c < NCW; c++) { P[c] = 0; R[c] = 0; }
         auto acc_pair = [&](Acc sn, Acc cs, int j) {
-          const Acc wr = (Acc)s_cw[j];
+          const Acc wr = kSplit ? (Acc)s_cwf[kSplit ? j : 0] : (Acc)s_cw[j];
+          if constexpr (TWO) {
+            const float4 v = *(const float4 *)&s_cf[cf_at(d, j, 0)];
+            const float ar[2] = {v.x, v.y}, ai[2] = {v.z, v.w};
 #pragma unroll
-          for (int c = 0; c < NCW; c++) {
-            const Acc ar = cf[c * n_om + j], ai = cf[(NCW + c) * n_om + j];
-            P[c] = fma(ar, cs, fma(ai, sn, P[c]));
-            R[c] = fma(wr, fma(ar, sn, -ai * cs), R[c]);
+            for (int c = 0; c < 2; c++) {
+              P[c] = fma(ar[c], cs, fma(ai[c], sn, P[c]));
+              R[c] = fma(wr, fma(ar[c], sn, -ai[c] * cs), R[c]);
+            }
+          } else {
+#pragma unroll
+            for (int c = 0; c < NCW; c++) {
+              const Acc ar = s_cf[cf_at(d, j, c)], ai = s_cf[cf_at(d, j, NCW + c)];
+              P[c] = fma(ar, cs, fma(ai, sn, P[c]));
+              R[c] = fma(wr, fma(ar, sn, -ai * cs), R[c]);
+            }
           }
         };
         if (rot) {
