@@ -60,7 +60,9 @@ def parse():
     ap.add_argument("--f64-particles", type=int, default=200)
     ap.add_argument("--no-f64", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--no-autotune", action="store_true")
+    ap.add_argument("--autotune", action="store_true",
+                    help="time candidate plans on a sample first (the library's own plan otherwise)")
+    ap.add_argument("--no-autotune", action="store_true", help="the library's own plan (the default)")
     ap.add_argument("--no-e2e", action="store_true", help="skip the drop-in call from host arrays")
     ap.add_argument("--cpu-particles", type=int, default=4)
     ap.add_argument("--cpu-grid", type=str, default="128,32,16")
@@ -248,7 +250,7 @@ def end_to_end(wl, precision):
 def plan_for(batch, grid, comp, precision, args):
     from dataclasses import asdict
     from synchrad_amd import autotune
-    if args.no_autotune:
+    if args.no_autotune or not args.autotune:
         return None, None
     t0 = time.perf_counter()
     plan, rows = autotune.tune(batch, grid, comp=comp, precision=precision)

@@ -6,8 +6,12 @@ also the number of steps per LDS tile, 64 per wave) -- on a sample of the real
 batch: up to 16 of its particles, truncated so that one candidate run costs
 about ``target_evals`` point-step evaluations.  The grid is the real grid, so
 the filter activity (which decides how much work each tile does) is the real
-one.  The fastest plan is cached in memory and, if ``SYNCHRAD_AMD_TUNE_CACHE``
-names a file, in that JSON file.
+one.  Large far-field launches (>= LARGE_EVALS evaluations) launch their tiles
+heaviest first and are planned with more particle slices; a sample that small
+would be timed in another regime, so for them the sample keeps full-length
+tracks of enough particles to be a large launch itself.  The fastest plan is
+cached in memory and, if ``SYNCHRAD_AMD_TUNE_CACHE`` names a file, in that JSON
+file.
 """
 from __future__ import annotations
 
@@ -23,6 +27,11 @@ from . import engine
 
 _cache: Dict[str, dict] = {}
 
+# far-field launches from this size on are ordered and sliced by the library
+# (its kOrderSliceEvals); their samples are at least LARGE_SAMPLE evaluations
+LARGE_EVALS = 1e12
+LARGE_SAMPLE = 1.2e12
+
 
 def candidates(mode: str, precision: str, uniform: bool) -> List[engine.Tuning]:
     T = engine.Tuning
@@ -31,8 +40,9 @@ def candidates(mode: str, precision: str, uniform: bool) -> List[engine.Tuning]:
     ms = (1, 2, 4) if mode == "far" else (1, 2)
     out = [T(sweep="direct", omega_per_lane=m, waves_per_block=w) for m in ms for w in (1, 2, 4)]
     if uniform:
+        # on linear grids the recurrence measured faster than every direct plan
         rms = (2, 4) if mode == "far" else (2,)
-        out += [T(sweep="recurrence", omega_per_lane=m, waves_per_block=w) for m in rms for w in (1, 2, 4)]
+        out = [T(sweep="recurrence", omega_per_lane=m, waves_per_block=w) for m in rms for w in (1, 2, 4)]
     return out
 
 
@@ -42,9 +52,10 @@ def _key(batch: engine.DeviceBatch, grid: engine.DeviceGrid, comp: str, precisio
                        bool(grid.omega_step), int(hb.n_snaps), int(min(hb.n_particles, 16))])
 
 
-def sample_batch(hb: engine.HostBatch, n_grid: int, target_evals: float) -> engine.HostBatch:
-    """The first <=16 particles, each cut to the same step budget."""
-    Np = min(hb.n_particles, 16)
+def sample_batch(hb: engine.HostBatch, n_grid: int, target_evals: float,
+                 max_particles: int = 16) -> engine.HostBatch:
+    """The first <= max_particles particles, each cut to the same step budget."""
+    Np = min(hb.n_particles, max_particles)
     lengths = np.diff(hb.offsets)[:Np]
     budget = max(8, int(target_evals / max(n_grid, 1) / max(Np, 1)))
     keep = np.minimum(lengths, budget)
@@ -86,7 +97,16 @@ def tune(batch: engine.DeviceBatch, grid: engine.DeviceGrid, comp: str = "total"
             pass
     if key in _cache:
         return engine.Tuning(**_cache[key]["plan"]), _cache[key]["timings"]
-    hb = sample_batch(batch.host, grid.n_grid, target_evals)
+    full = grid.n_grid * batch.host.n_intervals()
+    if grid.mode == "far" and full >= LARGE_EVALS:
+        # full-length tracks of as many particles as a large launch needs
+        lengths = np.diff(batch.host.offsets)
+        mean = max(float(lengths.mean()), 1.0) if lengths.size else 1.0
+        need = int(np.ceil(LARGE_SAMPLE / (grid.n_grid * mean)))
+        hb = sample_batch(batch.host, grid.n_grid, max(target_evals, LARGE_SAMPLE * 4),
+                          max_particles=max(16, need))
+    else:
+        hb = sample_batch(batch.host, grid.n_grid, target_evals)
     sb = engine.DeviceBatch(hb, grid.device)
     evals = grid.n_grid * hb.n_intervals()
     rows = []

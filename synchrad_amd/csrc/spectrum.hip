@@ -895,6 +895,8 @@ constexpr double kOrderMinEvals = 1e10;
 // used when the launch has at least kOrderEvalsPerTile2 * tiles^2 evaluations
 // (about 100 times the ranking's time at 5e12 evaluations/s)
 constexpr double kOrderEvalsPerTile2 = 4.5e3;
+// launches from this size on are planned with more particle slices (finer tiles)
+constexpr double kOrderSliceEvals = 1e12;
 constexpr float kOrderStepCost = SR_ORDER_STEP_COST;   // a step's fixed work, in units of a fully kept step's sweep
 
 // cost[tile] for every (direction, omega chunk), the same for each particle
@@ -942,13 +944,16 @@ __global__ void __launch_bounds__(256) k_tile_rank(const float *cost, int64_t n,
   int64_t rank = 0;
   for (int64_t j0 = 0; j0 < n; j0 += 256) {
     __syncthreads();
-    if (j0 + threadIdx.x < n) s_c[threadIdx.x] = cost[j0 + threadIdx.x];
+    // past n: -1, below every cost, so never counted
+    s_c[threadIdx.x] = (j0 + threadIdx.x < n) ? cost[j0 + threadIdx.x] : -1.f;
     __syncthreads();
-    const int m = (int)min((int64_t)256, n - j0);
-    for (int jj = 0; jj < m; jj++) {
+    int cnt = 0;
+#pragma unroll 16
+    for (int jj = 0; jj < 256; jj++) {
       const float cj = s_c[jj];
-      rank += (cj > ci || (cj == ci && j0 + jj < i)) ? 1 : 0;
+      cnt += (cj > ci || (cj == ci && j0 + jj < i)) ? 1 : 0;
     }
+    rank += cnt;
   }
   if (i < n) order[rank] = (int32_t)i;
 }
@@ -1029,7 +1034,8 @@ int allowed_m(int mode, int prec, int sweep, int m) {
 
 // Tile shape: prefer the largest omega run per block (T*M) that is mostly
 // used, then enough particle slices to put ~8 waves on each of the 1024 SIMDs.
-Plan make_plan(const sr_grid &g, const sr_opts &o, int64_t n_particles, uint32_t n_snaps) {
+// n_points: samples of the batch (0: unknown), for the size of the launch
+Plan make_plan(const sr_grid &g, const sr_opts &o, int64_t n_particles, uint32_t n_snaps, int64_t n_points = 0) {
   Plan pl;
   const int64_t n_om = g.n_omega;
   const int64_t n_dir = (int64_t)g.n_theta * g.n_phi;
@@ -1073,7 +1079,13 @@ Plan make_plan(const sr_grid &g, const sr_opts &o, int64_t n_particles, uint32_t
   if (o.slices > 0) {
     s = o.slices;
   } else {
-    const int64_t target = 8 * 1024;
+    // particle slices until the launch has ~8K waves; large far-field
+    // launches, whose tiles run heaviest first, until ~64K: smaller tiles
+    // leave a shorter tail (betatron beam +3 %, the uneven 20000-particle
+    // batch +20 % at 16 slices)
+    const double evals = (double)n_points * (double)(n_om * n_dir);
+    const int64_t target = (g.mode == SR_MODE_FAR && o.tile_order == 0 && evals >= kOrderSliceEvals)
+                               ? 64 * 1024 : 8 * 1024;
     while (s < kMaxSlices && base_waves * s < target && (int64_t)(s * 2) <= n_particles) s *= 2;
   }
   if (s > n_particles) s = (int)std::max<int64_t>(1, n_particles);
@@ -1398,7 +1410,7 @@ int sr_workspace_bytes(const sr_tracks *tracks, const sr_grid *grid, const sr_op
   *bytes = 0;
   if ((rc = check_tracks(tracks)) || (rc = check_grid(grid)) || (rc = check_opts(grid, opts))) return rc;
   if (tracks->n_particles == 0) return SR_OK;
-  const Plan pl = make_plan(*grid, *opts, tracks->n_particles, tracks->n_snaps);
+  const Plan pl = make_plan(*grid, *opts, tracks->n_particles, tracks->n_snaps, tracks->n_points);
   *bytes = scratch_bytes(pl, outputs_for(grid->mode, opts->comp),
                          (int64_t)grid->n_omega * grid->n_theta * grid->n_phi, tracks->n_snaps) +
            pre_bytes(grid->mode, tracks->n_points) + order_bytes(pl, *grid, *opts, tracks->n_points);
@@ -1427,7 +1439,7 @@ int sr_spectrum_batch(const sr_tracks *t, const sr_grid *g, const sr_opts *o, do
   kp.dt = t->dt;
   kp.n_points = t->n_points;
   fill_grid(kp, *g, *o, n_out, outs);
-  const Plan pl = make_plan(*g, *o, t->n_particles, t->n_snaps);
+  const Plan pl = make_plan(*g, *o, t->n_particles, t->n_snaps, t->n_points);
   slice_particles(kp, pl.slices, t->n_particles, t->offsets_host);
   return run(kp, pl, *g, *o, workspace, workspace_bytes, (hipStream_t)stream, true);
 }

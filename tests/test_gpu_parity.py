@@ -313,7 +313,7 @@ def test_autotuned_plan_matches_oracle():
     ax = wl.axes
     grid = engine.DeviceGrid("far", ax["omega"], ax["theta"], ax["phi"])
     plan, rows = autotune.tune(batch, grid, comp="total", precision="float", target_evals=1e8)
-    assert len(rows) >= 9 and all(r["ms"] > 0 for r in rows)
+    assert len(rows) >= 6 and all(r["ms"] > 0 for r in rows)
     out = engine.spectrum(batch, grid, comp="total", precision="float", tuning=plan)[0]
     got = out.cpu().numpy().swapaxes(-1, -3)
     assert rel_l2(got, ref["total"]) < SPLIT_TOL
