@@ -61,8 +61,8 @@ def parse():
     ap.add_argument("--no-f64", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--autotune", action="store_true",
-                    help="time candidate plans on a sample first (the library's own plan otherwise)")
-    ap.add_argument("--no-autotune", action="store_true", help="the library's own plan (the default)")
+                    help="always time candidate plans on a sample first")
+    ap.add_argument("--no-autotune", action="store_true", help="always the library's own plan")
     ap.add_argument("--no-e2e", action="store_true", help="skip the drop-in call from host arrays")
     ap.add_argument("--cpu-particles", type=int, default=4)
     ap.add_argument("--cpu-grid", type=str, default="128,32,16")
@@ -250,7 +250,10 @@ def end_to_end(wl, precision):
 def plan_for(batch, grid, comp, precision, args):
     from dataclasses import asdict
     from synchrad_amd import autotune
-    if args.no_autotune or not args.autotune:
+    if args.no_autotune:
+        return None, None
+    if not args.autotune and not autotune.representative(batch, grid):
+        # the autotuner's sample would run in another regime than the launch
         return None, None
     t0 = time.perf_counter()
     plan, rows = autotune.tune(batch, grid, comp=comp, precision=precision)

@@ -84,6 +84,14 @@ def time_plan(batch, grid, comp, precision, tun: engine.Tuning, reps: int = 2) -
     return e0.elapsed_time(e1) / reps
 
 
+def representative(batch: engine.DeviceBatch, grid: engine.DeviceGrid) -> bool:
+    """Whether a tuning sample ranks the plans as the full launch would.  Not
+    for large far-field launches: their tiles are ordered and sliced, and the
+    per-tile work of the full launch is many times the sample's (on the
+    betatron beam the sample's pick runs 8 % slower than the library's plan)."""
+    return not (grid.mode == "far" and grid.n_grid * batch.host.n_intervals() >= LARGE_EVALS)
+
+
 def tune(batch: engine.DeviceBatch, grid: engine.DeviceGrid, comp: str = "total",
          precision: str = "float", target_evals: float = 3e10, verbose: bool = False
          ) -> Tuple[engine.Tuning, List[dict]]:

@@ -1061,7 +1061,12 @@ Plan make_plan(const sr_grid &g, const sr_opts &o, int64_t n_particles, uint32_t
       // better: +8 % over 4 waves on the betatron beam), otherwise 4 waves
       const double mpref = (m == 2) ? 0.02 : 0.0;
       const double wpref = (pl.sweep == SWEEP_REC && g.mode == SR_MODE_FAR && w == 2) ? 0.03 : 0.0;
-      const double score = eff + 0.01 * w + mpref + wpref;
+      // launches too small to fill the GPU (one particle: no slices to add
+      // waves) prefer the plan with the most waves: 256 CUs hold ~8K waves
+      const double fill =
+          std::min(1.0, (double)(chunks * n_dir * w * std::min<int64_t>(std::max<int64_t>(n_particles, 1), kMaxSlices)) /
+                            8192.0);
+      const double score = eff * fill + 0.01 * w + mpref + wpref;
       if (score > best_score) { best_score = score; best_w = w; best_m = m; }
     }
   }
