@@ -165,6 +165,26 @@ def test_workspace_plan():
     assert rc == 0 and n == 0
 
 
+def test_large_far_launch_plan():
+    """A far-field batch of >= 1e12 evaluations is planned with particle slices
+    up to ~64K waves, as far as its tiles stay orderable, and launches them
+    heaviest first: 1e6 samples on 512 x 64 x 64 (2.1e12 evaluations), direct
+    sweep M = 2 x 4 waves (4096 tiles per slice) -> 4 slices, 16384 tiles."""
+    t = _tracks(n_particles=2000)
+    t.n_points = 1_000_000
+    n_grid = 512 * 64 * 64
+    pre = 6 * 8 * t.n_points
+    rc, n = _ws(t, _grid(), _opts())
+    assert rc == 0 and n == pre + 4 * n_grid * 8 + 8 * 16384
+    # index order: the small-launch plan (no slices on this grid), no order
+    rc, n = _ws(t, _grid(), _opts(tile_order=1))
+    assert rc == 0 and n == pre
+    # more evaluations would allow more slices, but the waves target stops at 4
+    t.n_points = 10_000_000
+    rc, n = _ws(t, _grid(), _opts())
+    assert rc == 0 and n == 6 * 8 * t.n_points + 4 * n_grid * 8 + 8 * 16384
+
+
 def test_per_particle_entry_points_validate():
     L = _abi.load()
     P = FAKE
