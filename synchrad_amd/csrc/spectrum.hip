@@ -148,16 +148,35 @@ struct KParams {
 // its own column, so no barrier guards them.
 template <int MODE, int M>
 constexpr size_t kahan_lds_bytes(int T) {
-  return (size_t)2 * (MODE == SR_MODE_FAR ? 4 : 12) * M * T * sizeof(float);
+  return (size_t)2 * (MODE == SR_MODE_FAR ? 4 : 6) * M * T * sizeof(float);
 }
 
-// Accumulator word of amplitude word c: far [A cos | A sin] with NA words
-// each (split: A.e_theta, A.e_phi; FP64: A_x, A_y, A_z); near
-// [v1 cos | v1 sin | v2 cos | v2 sin] with 3 words each.
-template <int MODE, int NA>
-__device__ constexpr int wcos(int c) { return MODE == SR_MODE_FAR ? c : 6 * (c / 3) + (c % 3); }
-template <int MODE, int NA>
-__device__ constexpr int wsin(int c) { return MODE == SR_MODE_FAR ? c + NA : 6 * (c / 3) + (c % 3) + 3; }
+__device__ __forceinline__ float fmx(float a, float b, float c) { return __builtin_fmaf(a, b, c); }
+__device__ __forceinline__ double fmx(double a, double b, double c) { return fma(a, b, c); }
+
+// One step's contribution to one frequency's sums acc[NC], from the step's
+// amplitude words a[NA] and e^{i phase} = cn + i sn.  Far: [A cos | A sin]
+// with NA words each (split: A.e_theta, A.e_phi; FP64: A_x, A_y, A_z).
+// Near: the field itself, F_c += (v2_c + i omega v1_c) e^{i phase} with
+// a = [v1 (3) | v2 (3)] and w = omega (kernel_nearfield.cl:87-91: c1 =
+// omega v1, c2 = v2), as [Re F (3) | Im F (3)].
+template <int MODE, int NA, typename X>
+__device__ __forceinline__ void accumulate(X *acc, const X *a, X cn, X sn, X w) {
+  if constexpr (MODE == SR_MODE_FAR) {
+#pragma unroll
+    for (int c = 0; c < NA; c++) {
+      acc[c] = fmx(a[c], cn, acc[c]);
+      acc[NA + c] = fmx(a[c], sn, acc[NA + c]);
+    }
+  } else {
+    const X wc = w * cn, ws = w * sn;
+#pragma unroll
+    for (int c = 0; c < 3; c++) {
+      acc[c] = fmx(a[3 + c], cn, fmx(-a[c], ws, acc[c]));
+      acc[3 + c] = fmx(a[3 + c], sn, fmx(a[c], wc, acc[3 + c]));
+    }
+  }
+}
 // Dynamic LDS after the Kahan state: the per-thread particle sums of the
 // single output when KParams::acc_lds is set, [M][T] doubles.
 template <int M>
@@ -198,7 +217,7 @@ struct LdsRows {
 // few spilled stage registers; the other recurrence variants spill heavily
 // at 4 and run at 2)
 #ifndef SR_REC_WAVES
-#define SR_REC_WAVES 2
+#define SR_REC_WAVES 3
 #endif
 #ifndef SR_REC_WAVES_FAR2
 #define SR_REC_WAVES_FAR2 4
@@ -258,7 +277,7 @@ k_spectrum(const KParams p) {
   // as its two components on e_theta, e_phi.  Far FP64: A_x, A_y, A_z as the
   // reference sums them.  Near: v1 = (b - n)/R and v2 = n/R^2.
   constexpr int NA = amp_words<MODE, PREC>();
-  constexpr int NC = 2 * NA;                           // real accumulators per omega
+  constexpr int NC = (MODE == SR_MODE_FAR) ? 2 * NA : 6;   // real accumulators per omega
   constexpr bool kCompact = kSplit && kRec && (SR_COMPACT != 0);
   constexpr int kMaxW = kMaxT / kWave;                 // waves per block
   constexpr int kMaxRuns = kMaxW * 4;                  // 64-omega runs per block (M <= 4)
@@ -336,8 +355,12 @@ k_spectrum(const KParams p) {
       om[k] = valid[k] ? p.omega[wi[k]] : 0.0;
     }
     double om_rev[M];
+    float omf[M];   // omega of the near field's v1 term (far: unused)
 #pragma unroll
-    for (int k = 0; k < M; k++) om_rev[k] = om[k] * kInv2Pi;
+    for (int k = 0; k < M; k++) {
+      om_rev[k] = om[k] * kInv2Pi;
+      omf[k] = (float)om[k];
+    }
     // rec sweep: the thread's base frequency in revolutions, the block's omega range
     const double om0_rev = om_rev[0];
     // range of |omega|/2pi over the block (linear grids: the run's end points)
@@ -441,11 +464,12 @@ k_spectrum(const KParams p) {
 #pragma unroll
           for (int c = 0; c < 3; c++) { re[c] = F[c]; im[c] = F[3 + c]; }
         } else {
-          // F = G2 + i omega G1 ; words: G1re 0..2, G1im 3..5, G2re 6..8, G2im 9..11
+          // near: the sums are the field, [Re F (3) | Im F (3)]
+          (void)omk;
 #pragma unroll
           for (int c = 0; c < 3; c++) {
-            re[c] = F[6 + c] - omk * F[3 + c];
-            im[c] = F[9 + c] + omk * F[c];
+            re[c] = F[c];
+            im[c] = F[3 + c];
           }
         }
         double v[kMaxOut];
@@ -525,11 +549,7 @@ k_spectrum(const KParams p) {
             cm = act ? cn : 0.f;
             sm = act ? sn : 0.f;
           }
-#pragma unroll
-          for (int c = 0; c < NA; c++) {
-            ps[k][wcos<MODE, NA>(c)] = __builtin_fmaf(a[c], cm, ps[k][wcos<MODE, NA>(c)]);
-            ps[k][wsin<MODE, NA>(c)] = __builtin_fmaf(a[c], sm, ps[k][wsin<MODE, NA>(c)]);
-          }
+          accumulate<MODE, NA>(ps[k], a, cm, sm, omf[k]);
           if (k + 1 < K) {
             const float cn2 = __builtin_fmaf(cn, er, -sn * ei);
             sn = __builtin_fmaf(cn, ei, sn * er);
@@ -634,19 +654,17 @@ k_spectrum(const KParams p) {
             if (kSplit) {
               float sn, cn;
               sincos_rev(ph, &sn, &cn);
+              float af[NA];
 #pragma unroll
-              for (int c = 0; c < NA; c++) {
-                ps[k][wcos<MODE, NA>(c)] = __builtin_fmaf((float)a[c], cn, ps[k][wcos<MODE, NA>(c)]);
-                ps[k][wsin<MODE, NA>(c)] = __builtin_fmaf((float)a[c], sn, ps[k][wsin<MODE, NA>(c)]);
-              }
+              for (int c = 0; c < NA; c++) af[c] = (float)a[c];
+              accumulate<MODE, NA>(ps[k], af, cn, sn, omf[k]);
             } else {
               double sn, cn;
               sincos_reduced(ph, &sn, &cn);
+              double ad[NA];
 #pragma unroll
-              for (int c = 0; c < NA; c++) {
-                fs[k][wcos<MODE, NA>(c)] = fma((double)a[c], cn, fs[k][wcos<MODE, NA>(c)]);
-                fs[k][wsin<MODE, NA>(c)] = fma((double)a[c], sn, fs[k][wsin<MODE, NA>(c)]);
-              }
+              for (int c = 0; c < NA; c++) ad[c] = (double)a[c];
+              accumulate<MODE, NA>(fs[k], ad, cn, sn, om[k]);
             }
           }
         }
@@ -721,7 +739,7 @@ k_spectrum(const KParams p) {
               st_amp[2] = (Amp)Az;
             }
           } else {
-            // kernel_nearfield.cl:67-88; omega of c1 is applied at the record
+            // kernel_nearfield.cl:67-88; omega of c1 = omega v1 is applied in the sums
             double b0x, b0y, b0z;
             if (p.pre) {
               const int64_t np = p.n_points;
