@@ -189,7 +189,8 @@ int sr_far_adjoint_batch(const sr_tracks *tracks, const sr_grid *grid, const sr_
  * (w dt^2 |F_p|^2 particle by particle in particle order, the records'
  * expressions of synchrad/kernel_farfield.cl:100-106, 208-222).  No
  * counterpart in the reference.  Same restrictions as the adjoint (far field,
- * one snapshot); the workspace holds the step data (6 * n_points doubles).
+ * one snapshot); the workspace holds the step data (6 * n_points doubles) and
+ * the tile launch order (8 bytes per tile).
  */
 int sr_far_fields_workspace_bytes(const sr_tracks *tracks, const sr_grid *grid, const sr_opts *opts,
                                   size_t *bytes);

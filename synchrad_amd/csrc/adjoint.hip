@@ -476,11 +476,13 @@ size_t adjoint_slab_words(const sr_tracks &t, const APlan &pl) {
 }
 
 // workspace: slabs, the fields of one chunk of particles (unless the caller
-// passes them), the step data
-size_t adjoint_ws(const sr_tracks &t, const sr_grid &g, const APlan &pl, bool own_fields) {
+// passes them), the step data, and (with own fields) the fields launches' tile
+// order
+size_t adjoint_ws(const sr_tracks &t, const sr_grid &g, const APlan &pl, bool own_fields, int32_t precision) {
   const size_t n_grid = (size_t)g.n_omega * g.n_theta * g.n_phi;
   return (adjoint_slab_words(t, pl) + (own_fields ? 6 * (size_t)pl.chunk * n_grid : 0) +
-          6 * (size_t)t.n_points) * sizeof(double);
+          6 * (size_t)t.n_points) * sizeof(double) +
+         (own_fields ? fields_order_space(g, precision, pl.chunk) : 0);
 }
 
 // the adjoint; fields_in = NULL: the fields are computed chunk by chunk in the
@@ -498,7 +500,7 @@ int adjoint_run(const sr_tracks *t, const sr_grid *g, const sr_opts *o, const do
   if (!gx || !gy || !gz || !gux || !guy || !guz || !gw) return fail(SR_ERR_INVALID, "a gradient pointer is NULL");
   const bool own = (fields_in == nullptr);
   const APlan pl = adjoint_plan(*t, *g, *o, !own);
-  const size_t need = adjoint_ws(*t, *g, pl, own);
+  const size_t need = adjoint_ws(*t, *g, pl, own, o->precision);
   if (!workspace || workspace_bytes < need)
     return fail(SR_ERR_WORKSPACE, "workspace of %zu bytes needed, %zu given", need, workspace_bytes);
   AParams p;
@@ -526,6 +528,8 @@ int adjoint_run(const sr_tracks *t, const sr_grid *g, const sr_opts *o, const do
   p.gx = gx; p.gy = gy; p.gz = gz; p.gux = gux; p.guy = guy; p.guz = guz; p.gw = gw;
   double *fields = (double *)workspace + adjoint_slab_words(*t, pl);
   double *pre = own ? fields + 6 * (size_t)pl.chunk * (size_t)p.n_grid : fields;
+  void *order_ws = own ? (void *)(pre + 6 * (size_t)t->n_points) : nullptr;
+  const size_t order_space = own ? fields_order_space(*g, o->precision, pl.chunk) : 0;
   p.F = own ? fields : fields_in;
   p.pre = pre;
   hipStream_t st = (hipStream_t)stream;
@@ -537,7 +541,7 @@ int adjoint_run(const sr_tracks *t, const sr_grid *g, const sr_opts *o, const do
     if (own) {
       const int64_t n_f = 6 * (pe - pb) * p.n_grid;
       hipLaunchKernelGGL(k_adjoint_clear, dim3((unsigned)((n_f + 255) / 256)), dim3(256), 0, st, fields, n_f);
-      if ((rc = spectrum_fields(*t, *g, o->precision, pb, pe, pre, fields, st))) return rc;
+      if ((rc = spectrum_fields(*t, *g, o->precision, pb, pe, pre, fields, st, order_ws, order_space))) return rc;
     }
     p.chunk_base = pb;
     p.chunk_n = pe - pb;
@@ -567,7 +571,7 @@ int sr_far_adjoint_workspace_bytes(const sr_tracks *tracks, const sr_grid *grid,
   int rc = adjoint_check(tracks, grid, opts);
   if (rc) return rc;
   if (tracks->n_particles == 0) return SR_OK;
-  *bytes = adjoint_ws(*tracks, *grid, adjoint_plan(*tracks, *grid, *opts, false), true);
+  *bytes = adjoint_ws(*tracks, *grid, adjoint_plan(*tracks, *grid, *opts, false), true, opts->precision);
   return SR_OK;
 }
 
@@ -583,7 +587,8 @@ int sr_far_fields_workspace_bytes(const sr_tracks *tracks, const sr_grid *grid, 
   *bytes = 0;
   int rc = adjoint_check(tracks, grid, opts);
   if (rc) return rc;
-  *bytes = 6 * (size_t)tracks->n_points * sizeof(double);
+  *bytes = 6 * (size_t)tracks->n_points * sizeof(double) +
+           fields_order_space(*grid, opts->precision, tracks->n_particles);
   return SR_OK;
 }
 
@@ -597,7 +602,8 @@ int sr_far_fields_batch(const sr_tracks *t, const sr_grid *g, const sr_opts *o, 
   if (outs)
     for (int c = 0; c < n_out; c++)
       if (!outs[c]) return fail(SR_ERR_INVALID, "outs[%d] is NULL", c);
-  const size_t need = 6 * (size_t)t->n_points * sizeof(double);
+  const size_t order_space = fields_order_space(*g, o->precision, t->n_particles);
+  const size_t need = 6 * (size_t)t->n_points * sizeof(double) + order_space;
   if (!workspace || workspace_bytes < need)
     return fail(SR_ERR_WORKSPACE, "workspace of %zu bytes needed, %zu given", need, workspace_bytes);
   hipStream_t st = (hipStream_t)stream;
@@ -606,7 +612,9 @@ int sr_far_fields_batch(const sr_tracks *t, const sr_grid *g, const sr_opts *o, 
   const int64_t n_f = 6 * t->n_particles * n_grid;
   hipLaunchKernelGGL(k_adjoint_clear, dim3((unsigned)((n_f + 255) / 256)), dim3(256), 0, st, fields, n_f);
   if ((rc = step_prep(*t, SR_MODE_FAR, pre, st))) return rc;
-  if ((rc = spectrum_fields(*t, *g, o->precision, 0, t->n_particles, pre, fields, st))) return rc;
+  if ((rc = spectrum_fields(*t, *g, o->precision, 0, t->n_particles, pre, fields, st,
+                            (void *)(pre + 6 * (size_t)t->n_points), order_space)))
+    return rc;
   if (outs)
     hipLaunchKernelGGL(k_fields_spectrum, dim3((unsigned)((n_grid + 255) / 256)), dim3(256), 0, st, t->offsets,
                        t->weight, t->dt, t->n_particles, n_grid, o->comp, (const double *)fields, outs[0],
@@ -623,7 +631,7 @@ int sr_far_adjoint_fields_workspace_bytes(const sr_tracks *tracks, const sr_grid
   int rc = adjoint_check(tracks, grid, opts);
   if (rc) return rc;
   if (tracks->n_particles == 0) return SR_OK;
-  *bytes = adjoint_ws(*tracks, *grid, adjoint_plan(*tracks, *grid, *opts, true), false);
+  *bytes = adjoint_ws(*tracks, *grid, adjoint_plan(*tracks, *grid, *opts, true), false, opts->precision);
   return SR_OK;
 }
 

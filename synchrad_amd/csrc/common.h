@@ -38,9 +38,13 @@ int step_prep(const sr_tracks &t, int32_t mode, double *pre, hipStream_t st);
 // Adds the fields of particles [p_begin, p_end) (one snapshot record each) into
 // F[6][p_end - p_begin][n_grid] with the tile kernel's own plan for `precision`
 // (spectrum.hip); pre is step_prep's output or nullptr.  F must be cleared by
-// the caller; stream-ordered.
+// the caller; stream-ordered.  order_ws (fields_order_space bytes for the
+// largest chunk, or nullptr) lets large launches run their tiles heaviest first.
 int spectrum_fields(const sr_tracks &t, const sr_grid &g, int32_t precision, int64_t p_begin, int64_t p_end,
-                    const double *pre, double *F, hipStream_t st);
+                    const double *pre, double *F, hipStream_t st, void *order_ws = nullptr,
+                    size_t order_ws_bytes = 0);
+// workspace for the tile order of a fields launch of n_particles particles
+size_t fields_order_space(const sr_grid &g, int32_t precision, int64_t n_particles);
 
 // ---------------------------------------------------------------------------
 // device helpers

@@ -909,10 +909,10 @@ constexpr int kOrderSamples = SR_ORDER_SAMPLES;
 constexpr int64_t kOrderMinTiles = 2048;
 constexpr int64_t kOrderMaxTiles = 65536;
 constexpr double kOrderMinEvals = 1e10;
-// the ranking compares every pair of tiles (0.6 ms for 8192 tiles); it is
+// the ranking compares every pair of tiles (2.3 ms for 32768 tiles); it is
 // used when the launch has at least kOrderEvalsPerTile2 * tiles^2 evaluations
 // (about 100 times the ranking's time at 5e12 evaluations/s)
-constexpr double kOrderEvalsPerTile2 = 4.5e3;
+constexpr double kOrderEvalsPerTile2 = 1.2e3;
 // launches from this size on are planned with more particle slices (finer tiles)
 constexpr double kOrderSliceEvals = 1e12;
 constexpr float kOrderStepCost = SR_ORDER_STEP_COST;   // a step's fixed work, in units of a fully kept step's sweep
@@ -1253,8 +1253,10 @@ size_t order_bytes(const Plan &pl, const sr_grid &g, const sr_opts &o, int64_t n
 
 // want_pre: compute the step data into the workspace (batch calls); the
 // single-particle entry points have no workspace and compute it in the stage
+// fields_order: bytes of the tile order for a fields launch (its workspace is
+// only the order; batch calls size theirs from the plan)
 int run(KParams &kp, const Plan &pl, const sr_grid &g, const sr_opts &o, void *ws, size_t ws_bytes,
-        hipStream_t st, bool want_pre = false) {
+        hipStream_t st, bool want_pre = false, size_t fields_order = 0) {
   const int threads = kWave * pl.waves;
   const int64_t run = (int64_t)threads * pl.M;
   kp.n_chunks = (uint32_t)((g.n_omega + run - 1) / run);
@@ -1264,7 +1266,7 @@ int run(KParams &kp, const Plan &pl, const sr_grid &g, const sr_opts &o, void *w
   // per-particle outputs are disjoint: slices write them in place
   const size_t need = kp.part_stride ? 0 : scratch_bytes(pl, kp.n_out, kp.n_grid, kp.n_snaps);
   const size_t need_pre = want_pre ? pre_bytes(g.mode, kp.n_points) : 0;
-  const size_t need_order = want_pre ? order_bytes(pl, g, o, kp.n_points) : 0;
+  const size_t need_order = want_pre ? order_bytes(pl, g, o, kp.n_points) : fields_order;
   hipError_t e;
   if (need + need_pre + need_order && (!ws || ws_bytes < need + need_pre + need_order))
     return fail(SR_ERR_WORKSPACE, "workspace of %zu bytes needed, %zu given", need + need_pre + need_order,
@@ -1386,8 +1388,30 @@ int step_prep(const sr_tracks &t, int32_t mode, double *pre, hipStream_t st) {
   return e == hipSuccess ? SR_OK : fail(SR_ERR_HIP, "step data launch: %s", hipGetErrorString(e));
 }
 
+namespace {
+// samples of particles [p_begin, p_end): exact with the host offsets, else
+// the batch's mean
+int64_t chunk_points(const sr_tracks &t, int64_t p_begin, int64_t p_end) {
+  if (t.offsets_host) return t.offsets_host[p_end] - t.offsets_host[p_begin];
+  return t.n_particles ? t.n_points * (p_end - p_begin) / t.n_particles : 0;
+}
+sr_opts fields_opts(int32_t precision) {
+  sr_opts o;
+  memset(&o, 0, sizeof(o));
+  o.comp = kCompField;
+  o.precision = precision;
+  return o;
+}
+}  // namespace
+
+size_t fields_order_space(const sr_grid &g, int32_t precision, int64_t n_particles) {
+  const Plan pl = make_plan(g, fields_opts(precision), std::max<int64_t>(n_particles, 1), 1);
+  if (pl.blocks > kOrderMaxTiles) return 0;
+  return ((size_t)pl.blocks * (sizeof(float) + sizeof(int32_t)) + 255) / 256 * 256;
+}
+
 int spectrum_fields(const sr_tracks &t, const sr_grid &g, int32_t precision, int64_t p_begin, int64_t p_end,
-                    const double *pre, double *F, hipStream_t st) {
+                    const double *pre, double *F, hipStream_t st, void *order_ws, size_t order_ws_bytes) {
   KParams kp;
   memset(&kp, 0, sizeof(kp));
   kp.offsets = t.offsets;
@@ -1401,10 +1425,7 @@ int spectrum_fields(const sr_tracks &t, const sr_grid &g, int32_t precision, int
   kp.dt = t.dt;
   kp.n_points = t.n_points;
   kp.pre = pre;
-  sr_opts o;
-  memset(&o, 0, sizeof(o));
-  o.comp = kCompField;
-  o.precision = precision;
+  const sr_opts o = fields_opts(precision);
   const int64_t np = p_end - p_begin;
   const int64_t n_grid = (int64_t)g.n_omega * g.n_theta * g.n_phi;
   double *outs[kMaxOut];
@@ -1414,7 +1435,11 @@ int spectrum_fields(const sr_tracks &t, const sr_grid &g, int32_t precision, int
   for (int q = 0; q <= pl.slices; q++) kp.slice_lo[q] = p_begin + np * q / pl.slices;
   kp.part_base = p_begin;
   kp.part_stride = n_grid;
-  return run(kp, pl, g, o, nullptr, 0, st);
+  // heaviest tiles first where the launch is large enough and the caller
+  // gave room for the order
+  size_t ob = order_ws ? order_bytes(pl, g, o, chunk_points(t, p_begin, p_end)) : 0;
+  if (ob > order_ws_bytes) ob = 0;
+  return run(kp, pl, g, o, order_ws, order_ws_bytes, st, false, ob);
 }
 
 }  // namespace sr
