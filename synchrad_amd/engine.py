@@ -313,7 +313,8 @@ def spectrum(batch: DeviceBatch, grid: DeviceGrid, comp: str = "total", precisio
     Stream-ordered on the current (or given) torch stream; no synchronisation.
 
     ``tuning``: None (the library's own plan), a :class:`Tuning`, or "auto"
-    (the autotuner's plan for this batch and grid, timed once and cached).
+    (the autotuner's plan for this batch and grid, timed once and cached;
+    the library's plan for launches the autotuner's samples cannot rank).
     ``workspace``: a preallocated uint8 device tensor to use instead of
     allocating one (needed under graph capture)."""
     L = _abi.load()
@@ -322,7 +323,9 @@ def spectrum(batch: DeviceBatch, grid: DeviceGrid, comp: str = "total", precisio
             raise ValueError("tuning must be None, a Tuning or 'auto'")
         from . import autotune
         prec_name = precision if isinstance(precision, str) else ("double" if precision == 0 else "float")
-        tuning, _ = autotune.tune(batch, grid, comp=comp, precision=prec_name)
+        # launches no sample can rank (large far-field ones) keep the library's plan
+        tuning = autotune.tune(batch, grid, comp=comp, precision=prec_name)[0] \
+            if autotune.representative(batch, grid) else None
     if comp not in _abi.COMPS:
         raise ValueError(f"unknown comp {comp!r}")
     prec = PRECISIONS[precision] if isinstance(precision, str) else int(precision)
