@@ -417,7 +417,8 @@ class SpectrumGraph:
         self.batch, self.grid, self.comp, self.precision = batch, grid, comp, precision
         if isinstance(tuning, str):
             from . import autotune
-            tuning, _ = autotune.tune(batch, grid, comp=comp, precision=precision)
+            tuning = autotune.tune(batch, grid, comp=comp, precision=precision)[0] \
+                if autotune.representative(batch, grid) else None
         self.tuning = tuning or Tuning()
         L = _abi.load()
         n_out = L.sr_num_outputs(_abi.MODES[grid.mode], _abi.COMPS[comp])
