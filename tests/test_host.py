@@ -194,3 +194,21 @@ def test_bench_refuses_more_than_one_gpu():
     assert r.returncode == 2
     msg = json.loads(r.stdout.strip().splitlines()[-1])
     assert "one GPU" in msg["error"]
+
+
+def test_autotuner_leaves_large_far_launches_to_the_library_plan():
+    """Far-field launches of >= 1e12 evaluations are ordered and sliced by the
+    library, and no tuning sample ranks their plans; smaller launches and the
+    near field are tuned."""
+    from types import SimpleNamespace
+    from synchrad_amd import autotune
+
+    def case(mode, n_grid, intervals):
+        grid = SimpleNamespace(mode=mode, n_grid=n_grid)
+        batch = SimpleNamespace(host=SimpleNamespace(n_intervals=lambda: intervals))
+        return autotune.representative(batch, grid)
+
+    assert not case("far", 512 * 64 * 64, 2000 * 7999)     # the betatron beam, 3.4e13
+    assert case("far", 256 * 32 * 32, 1599)                # one undulator electron, 4.2e8
+    assert case("near", 128 * 64 * 64, 500 * 3999)         # near field: not reordered
+    assert case("far", 512 * 64 * 64, 100) and not case("far", 512 * 64 * 64, 500_000)
