@@ -113,6 +113,11 @@ __device__ __forceinline__ void sincos_reduced(double ph, double *s, double *c) 
 
 // sin/cos of 2*pi*rv from an FP64 count of revolutions: the fraction is taken
 // exactly in FP64, then the FP32 hardware units (input in revolutions) run.
+// component q (0..3, a compile-time constant after unrolling) of a float4
+__device__ __forceinline__ float lane4(const float4 &v, int q) {
+  return q == 0 ? v.x : q == 1 ? v.y : q == 2 ? v.z : v.w;
+}
+
 __device__ __forceinline__ void sincos_rev(double rv, float *s, float *c) {
   const float f = (float)__builtin_amdgcn_fract(rv);
   *s = __builtin_amdgcn_sinf(f);

@@ -238,6 +238,10 @@ struct LdsRows {
 #ifndef SR_DIRECT_WAVES
 #define SR_DIRECT_WAVES 3
 #endif
+// split direct sweep in 4-step groups
+#ifndef SR_DIRECT_GROUP
+#define SR_DIRECT_GROUP 1
+#endif
 #ifndef SR_F64_WAVES
 #define SR_F64_WAVES 4
 #endif
@@ -669,6 +673,42 @@ k_spectrum(const KParams p) {
           }
         }
       };
+      // steps [j, jend) of the direct sweep.  Split: single steps up to a
+      // multiple of 4, then groups of 4 read with 16-byte LDS reads (each
+      // frequency still skipped where the filter rejects it), then the tail
+      auto direct_sweep = [&](int &j, int jend) {
+        if constexpr (kSplit && SR_DIRECT_GROUP) {
+          const int jg = min(jend, (j + 3) & ~3);
+          for (; j < jg; j++) direct_step(j);
+          for (; j + 4 <= jend; j += 4) {
+            const double2 t01 = *(const double2 *)&s_tau[j];
+            const double2 t23 = *(const double2 *)&s_tau[j + 2];
+            const double2 h01 = *(const double2 *)&s_thr[j];
+            const double2 h23 = *(const double2 *)&s_thr[j + 2];
+            float4 av[NA];
+#pragma unroll
+            for (int c = 0; c < NA; c++) av[c] = *(const float4 *)&s_amp[c][j];
+            const double tq[4] = {t01.x, t01.y, t23.x, t23.y};
+            const double hq[4] = {h01.x, h01.y, h23.x, h23.y};
+#pragma unroll
+            for (int q = 0; q < 4; q++) {
+              float a[NA];
+#pragma unroll
+              for (int c = 0; c < NA; c++) a[c] = lane4(av[c], q);
+#pragma unroll
+              for (int k = 0; k < M; k++) {
+                if (valid[k] && fabs(om_rev[k]) < hq[q]) {
+                  float sn, cn;
+                  sincos_rev(om_rev[k] * tq[q], &sn, &cn);
+                  accumulate<MODE, NA>(ps[k], a, cn, sn, omf[k]);
+                }
+              }
+            }
+          }
+        }
+#pragma unroll 2
+        for (; j < jend; j++) direct_step(j);
+      };
       // split: Kahan fold of the segment's partial sums into the LDS S, C
       auto kahan_fold = [&]() {
         if (!kSplit) return;
@@ -858,8 +898,7 @@ k_spectrum(const KParams p) {
               else
                 rec_sweep(IC<M>(), IC<1>(), j, jend);
             } else {
-#pragma unroll 2
-              for (; j < jend; j++) direct_step(j);
+              direct_sweep(j, jend);
             }
             kahan_fold();
             if (rec_here) {
