@@ -391,3 +391,24 @@ def test_heaviest_tiles_first_changes_no_result(precision, comp, slices, n_parti
     torch.cuda.synchronize()
     for x, y in zip(a, b):
         assert torch.equal(x, y)
+
+
+@pytest.mark.parametrize("descending", [False, True])
+def test_heaviest_tiles_first_partial_chunk_and_descending_grid(descending):
+    """The order also covers grids whose last omega chunk is partial (300
+    omegas over 256-omega chunks: 2048 directions x 2 chunks) and descending
+    grids (the cost estimate uses the chunk's |omega| range); results equal
+    the index-order launch bit for bit."""
+    wl = workloads.config2(n_particles=20, n_steps=8000, grid=(300, 32, 64))
+    ax = wl.axes
+    om = ax["omega"][::-1].copy() if descending else ax["omega"]
+    grid = engine.DeviceGrid("far", om, ax["theta"], ax["phi"])
+    batch = engine.DeviceBatch(wl.batch)
+    on = engine.Tuning(slices=1, omega_per_lane=2, waves_per_block=2)
+    off = engine.Tuning(slices=1, omega_per_lane=2, waves_per_block=2, tile_order=1)
+    assert engine.workspace_bytes(batch, grid, "total", "float", on) > \
+        engine.workspace_bytes(batch, grid, "total", "float", off)
+    a = engine.spectrum(batch, grid, comp="total", precision="float", tuning=on)[0]
+    b = engine.spectrum(batch, grid, comp="total", precision="float", tuning=off)[0]
+    torch.cuda.synchronize()
+    assert torch.equal(a, b)
