@@ -238,6 +238,10 @@ struct LdsRows {
 #ifndef SR_DIRECT_WAVES
 #define SR_DIRECT_WAVES 3
 #endif
+// split stage: hardware FP64 reciprocal instead of IEEE division
+#ifndef SR_SPLIT_RCP
+#define SR_SPLIT_RCP 1
+#endif
 // split direct sweep in 4-step groups
 #ifndef SR_DIRECT_GROUP
 #define SR_DIRECT_GROUP 1
@@ -757,7 +761,15 @@ k_spectrum(const KParams p) {
             }
             double c1 = ax * gx + ay * gy + az * gz;
             double c2 = 1.0 - (mx * gx + my * gy + mz * gz);
-            c2 = 1.0 / c2;
+            // split: the amplitude ends in FP32, so the hardware reciprocal
+            // (relative error <= 4.6e-8 measured on gfx950) with one Newton
+            // step (~1e-15) replaces the IEEE division
+            if (kSplit && SR_SPLIT_RCP) {
+              const double r = __builtin_amdgcn_rcp(c2);
+              c2 = fma(r, fma(-c2, r, 1.0), r);
+            } else {
+              c2 = 1.0 / c2;
+            }
             c1 = c1 * c2 * c2;
             double Ax = c1 * (gx - mx) - c2 * ax;
             double Ay = c1 * (gy - my) - c2 * ay;
@@ -810,7 +822,19 @@ k_spectrum(const KParams p) {
           if (kSplit) {
             // |omega tau_s - omega tau_{s-1}| < pi  <=>  |omega|/2pi < (1/2)/|tau_s - tau_{s-1}|
             // (phase_{-1} = 0 at the first step); 1/0 gives +inf: always active
-            const double thr = 0.5 / fabs(tau - tau_prev);
+            // (split: the hardware reciprocal with one Newton step, as for c2;
+            // the raw one, 4.6e-8, moved enough boundary decisions to raise the
+            // largest element error of the betatron spectrum from 1e-7 to 6e-7)
+            // (d = 0: +inf, always active, as the division gives; the Newton
+            // step alone would make it fma(-0, inf, 1) = NaN)
+            double thr;
+            if (SR_SPLIT_RCP) {
+              const double d = fabs(tau - tau_prev);
+              const double r = __builtin_amdgcn_rcp(d);
+              thr = (d == 0.0) ? INFINITY : 0.5 * fma(r, fma(-d, r, 1.0), r);
+            } else {
+              thr = 0.5 / fabs(tau - tau_prev);
+            }
             st_thr = thr;
             step_on = thr > w_hi;
           }

@@ -412,3 +412,35 @@ def test_heaviest_tiles_first_partial_chunk_and_descending_grid(descending):
     b = engine.spectrum(batch, grid, comp="total", precision="float", tuning=off)[0]
     torch.cuda.synchronize()
     assert torch.equal(a, b)
+
+
+@pytest.mark.parametrize("sweep", ["direct", "recurrence"])
+def test_zero_phase_step_is_kept(sweep):
+    """A step whose retarded time does not advance (tau_s = tau_{s-1}: the
+    track moves exactly dt along z per step, seen from theta = 0) passes the
+    filter |phase_s - phase_{s-1}| = 0 < pi, so its threshold is +inf in the
+    split path; the reciprocal there must not turn it into NaN and drop it."""
+    dt = 1e-6
+    n = 400
+    s = np.arange(n, dtype=np.float64)
+    x = 1e-7 * np.sin(0.05 * s)
+    y = np.zeros(n)
+    z = s * dt                                   # tau = t - z = 0 exactly on the axis
+    ux = 0.5 * np.cos(0.05 * s)
+    uy = np.zeros(n)
+    uz = np.full(n, 50.0)
+    tr = [[x, y, z, ux, uy, uz, 1.0, 0]]
+    axes = {"omega": np.linspace(1e5, 4e5, 64), "theta": np.linspace(0.0, 0.01, 4),
+            "phi": np.linspace(0.0, 2 * np.pi, 4, endpoint=False)}
+    ref = oracle.spectrum("far", "total", tr, axes, dt)["total"]
+    batch = engine.DeviceBatch(engine.pack_tracks(tr, dt=dt))
+    grid = engine.DeviceGrid("far", axes["omega"], axes["theta"], axes["phi"])
+    out = engine.spectrum(batch, grid, comp="total", precision="float",
+                          tuning=engine.Tuning(sweep=sweep))[0]
+    got = out.cpu().numpy().swapaxes(-1, -3)
+    assert np.isfinite(got).all()
+    # the on-axis column, where every step has tau_s = tau_{s-1}
+    r_axis = rel_l2(got[..., 0, :], ref[..., 0, :])
+    r = rel_l2(got, ref)
+    print(f"zero-phase steps, {sweep}: on-axis rel-L2 {r_axis:.3e}, all {r:.3e}", flush=True)
+    assert r_axis <= SPLIT_TOL and r <= SPLIT_TOL
