@@ -12,7 +12,8 @@ Differences, all deliberate:
   split-precision kernels: the phase and its reduction in FP64, amplitudes,
   sin/cos and compensated sums in FP32.  Host axes, tracks and ``timeStep``
   stay float64 in both modes (the reference casts them to float32 in its float
-  mode, which costs several radians of phase at the BASELINE sizes).
+  mode, which costs several radians of phase on the betatron beam's 8000-step
+  tracks).
 * All particles go to the device as one CSR batch and are integrated by one
   stream-ordered call, instead of one launch per particle.
 * String options are compared with ``==`` (the reference uses ``is``).
