@@ -1165,20 +1165,21 @@ Plan make_plan(const sr_grid &g, const sr_opts &o, int64_t n_particles, uint32_t
   if (o.slices > 0) {
     s = o.slices;
   } else {
-    // particle slices until the launch has ~8K waves; large far-field
-    // launches, whose tiles run heaviest first, until ~64K: smaller tiles
-    // leave a shorter tail (betatron beam +3 %, the uneven 20000-particle
-    // batch +20 % at 16 slices)
-    // (only as far as the tiles stay orderable: more slices without the order
-    // would lengthen the tail instead)
+    // particle slices until the launch has ~8K waves; large launches until
+    // ~64K: smaller tiles leave a shorter tail (betatron beam +3 %, the uneven
+    // 20000-particle batch +20 % at 16 slices, the near-field screen +6 % at
+    // 12 of equal-cost tiles).  Far-field tiles run heaviest first, and only
+    // as far as they stay orderable: more slices without the order would
+    // lengthen the tail instead.
     const double evals = (double)n_points * (double)(n_om * n_dir);
-    const bool big = g.mode == SR_MODE_FAR && o.tile_order == 0 && evals >= kOrderSliceEvals;
+    const bool far_ordered = g.mode == SR_MODE_FAR && o.tile_order == 0;
+    const bool big = evals >= kOrderSliceEvals && (g.mode != SR_MODE_FAR || far_ordered);
     const int64_t target = big ? 64 * 1024 : 8 * 1024;
     auto orderable = [&](int64_t tiles) {
       return tiles <= kOrderMaxTiles && evals >= kOrderEvalsPerTile2 * (double)tiles * (double)tiles;
     };
     while (s < kMaxSlices && base_waves * s < target && (int64_t)(s * 2) <= n_particles &&
-           (!big || orderable(chunks * n_dir * s * 2)))
+           (!(big && far_ordered) || orderable(chunks * n_dir * s * 2)))
       s *= 2;
   }
   if (s > n_particles) s = (int)std::max<int64_t>(1, n_particles);
