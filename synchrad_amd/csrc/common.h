@@ -13,7 +13,7 @@ namespace sr {
 
 constexpr int kWave = 64;
 constexpr int kMaxT = 256;        // threads per block = steps per LDS tile
-constexpr int kMaxSlices = 16;
+constexpr int kMaxSlices = 32;
 constexpr int kMaxOut = 6;
 
 constexpr double kPi = 3.141592653589793;
