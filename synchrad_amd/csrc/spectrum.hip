@@ -298,6 +298,17 @@ k_spectrum(const KParams p) {
   __shared__ int s_hist[kCompact ? kMaxRuns * kMaxW : 1];
   __shared__ int s_off[kCompact ? kMaxRuns * kMaxW : 1];
   __shared__ int s_cntb[kCompact ? kMaxRuns + 1 : 1];
+  // FP64 sin/cos table (sincos_table)
+  constexpr bool kTab = !kSplit && (SR_F64_TABLE != 0);
+  __shared__ double2 s_sct[kTab ? kSinCosTab : 1];
+  if (kTab) {
+    for (int i = threadIdx.x; i < kSinCosTab; i += blockDim.x) {
+      double sv, cv;
+      sincospi((double)i / (kSinCosTab / 2), &sv, &cv);
+      s_sct[i] = make_double2(sv, cv);
+    }
+    __syncthreads();
+  }
 
   const int T = blockDim.x;
   const int tid = threadIdx.x;
@@ -668,7 +679,10 @@ k_spectrum(const KParams p) {
               accumulate<MODE, NA>(ps[k], af, cn, sn, omf[k]);
             } else {
               double sn, cn;
-              sincos_reduced(ph, &sn, &cn);
+              if (kTab)
+                sincos_table(ph, s_sct, &sn, &cn);
+              else
+                sincos_reduced(ph, &sn, &cn);
               double ad[NA];
 #pragma unroll
               for (int c = 0; c < NA; c++) ad[c] = (double)a[c];
