@@ -98,7 +98,17 @@ __global__ void __launch_bounds__(kMaxT) k_adjoint(const AParams p) {
   // read in one access); FP64 [dir_batch][NW][n_omega] (word-major keeps the
   // FP64 sweep within 128 registers)
   Acc *const s_cf = (Acc *)s_dyn_raw;
-
+  // FP64 sin/cos table (sincos_table)
+  constexpr bool kTab = !kSplit && (SR_F64_TABLE != 0);
+  __shared__ double2 s_sct[kTab ? kSinCosTab : 1];
+  if (kTab) {
+    for (int i = threadIdx.x; i < kSinCosTab; i += blockDim.x) {
+      double sv, cv;
+      sincospi((double)i / (kSinCosTab / 2), &sv, &cv);
+      s_sct[i] = make_double2(sv, cv);
+    }
+    __syncthreads();
+  }
 
   const int T = blockDim.x;
   const int tid = threadIdx.x;
@@ -277,7 +287,10 @@ __global__ void __launch_bounds__(kMaxT) k_adjoint(const AParams p) {
               sn = s32; cs = c32;
             } else {
               double s64, c64;
-              sincos_reduced(ph, &s64, &c64);
+              if (kTab)
+                sincos_table(ph, s_sct, &s64, &c64);
+              else
+                sincos_reduced(ph, &s64, &c64);
               sn = s64; cs = c64;
             }
             acc_pair(sn, cs, j);
