@@ -101,13 +101,22 @@ __global__ void __launch_bounds__(kMaxT) k_adjoint(const AParams p) {
   // FP64 sin/cos table (sincos_table)
   constexpr bool kTab = !kSplit && (SR_F64_TABLE != 0);
   __shared__ double2 s_sct[kTab ? kSinCosTab : 1];
+  __shared__ double s_wmax[kTab ? kMaxT / kWave : 1];
+  // and the largest |omega| of the grid: a step whose |tau| * om_abs_max
+  // reaches kTabMaxPhase / 2 takes the table's wide-phase form
+  double om_abs_max = 0.0;
   if (kTab) {
     for (int i = threadIdx.x; i < kSinCosTab; i += blockDim.x) {
       double sv, cv;
       sincospi((double)i / (kSinCosTab / 2), &sv, &cv);
       s_sct[i] = make_double2(sv, cv);
     }
+    double m = 0.0;
+    for (int64_t i = threadIdx.x; i < (int64_t)p.n_omega; i += blockDim.x) m = fmax(m, fabs(p.omega[i]));
+    for (int o = kWave / 2; o > 0; o >>= 1) m = fmax(m, __shfl_xor(m, o));
+    if (threadIdx.x % kWave == 0) s_wmax[threadIdx.x / kWave] = m;
     __syncthreads();
+    for (int w = 0; w < (int)(blockDim.x / kWave); w++) om_abs_max = fmax(om_abs_max, s_wmax[w]);
   }
 
   const int T = blockDim.x;
@@ -271,30 +280,38 @@ __global__ void __launch_bounds__(kMaxT) k_adjoint(const AParams p) {
           }
         } else {
           const double thr = kSplit ? 0.5 / fabs(tau - tau_prev) : 0.0;
-          for (int j = 0; j < n_om; j++) {
-            const double w = s_cw[j];
-            const double ph = w * tau;
-            if (kSplit) {
-              if (!(fabs(w) < thr)) continue;   // the forward's threshold form
-            } else {
-              const double php = (s > 0) ? w * tau_prev : 0.0;
-              if (!(fabs(ph - php) < kPi)) continue;   // kernel_farfield.cl:69-72, exactly
+          // WIDE: a step of the wave may reach the sin/cos table's limit (a
+          // compile-time flag, so the usual loop is the table form alone)
+          auto om_sweep = [&](auto Wc) {
+            for (int j = 0; j < n_om; j++) {
+              const double w = s_cw[j];
+              const double ph = w * tau;
+              if (kSplit) {
+                if (!(fabs(w) < thr)) continue;   // the forward's threshold form
+              } else {
+                const double php = (s > 0) ? w * tau_prev : 0.0;
+                if (!(fabs(ph - php) < kPi)) continue;   // kernel_farfield.cl:69-72, exactly
+              }
+              Acc sn, cs;
+              if (kSplit) {
+                float s32, c32;
+                sincos_rev(ph, &s32, &c32);
+                sn = s32; cs = c32;
+              } else {
+                double s64, c64;
+                if (kTab)
+                  sincos_table(ph, s_sct, decltype(Wc)::value != 0, &s64, &c64);
+                else
+                  sincos_reduced(ph, &s64, &c64);
+                sn = s64; cs = c64;
+              }
+              acc_pair(sn, cs, j);
             }
-            Acc sn, cs;
-            if (kSplit) {
-              float s32, c32;
-              sincos_rev(ph, &s32, &c32);
-              sn = s32; cs = c32;
-            } else {
-              double s64, c64;
-              if (kTab)
-                sincos_table(ph, s_sct, &s64, &c64);
-              else
-                sincos_reduced(ph, &s64, &c64);
-              sn = s64; cs = c64;
-            }
-            acc_pair(sn, cs, j);
-          }
+          };
+          if (kTab && __ballot(fabs(tau) * om_abs_max >= 0.5 * kTabMaxPhase) != 0)
+            om_sweep(std::integral_constant<int, 1>());
+          else
+            om_sweep(std::integral_constant<int, 0>());
         }
         double gA[3], gR[3];
         if constexpr (TWO) {

@@ -114,7 +114,9 @@ __device__ __forceinline__ void sincos_reduced(double ph, double *s, double *c) 
 // FP64 sin and cos of a phase from a table of sin/cos(i pi/128), i = 0..255
 // (kSinCosTab entries, held in LDS by the caller).  k = rint(128 ph / pi) comes
 // from one FMA with 1.5 * 2^52 added (the sum's low mantissa bits are then k,
-// so k mod 256 is read off its low word; valid for |ph| < 2^44); the remainder
+// so k mod 256 is read off its low word, for |ph| < kTabMaxPhase; when `wide`
+// (uniform: the caller's bound on |ph| reaches kTabMaxPhase / 2) the phase is
+// first brought into [-pi, pi] with 2 pi split in two); the remainder
 // r = ph - k pi/128 is formed with pi/128 split in two as above, |r| <= pi/256;
 // sin r to r^5 and cos r to r^6 (first omitted terms below 1e-17 there); then
 // sin ph = S_k cos r + C_k sin r and cos ph = C_k cos r - S_k sin r.  Against
@@ -127,8 +129,13 @@ constexpr double kTabPerRad = 40.74366543152521;      // 128 / pi
 constexpr double kTabStepHi = 0.02454369260617026;    // fl(pi / 128)
 constexpr double kTabStepLo = 9.567553118338697e-19;  // pi / 128 - kTabStepHi
 constexpr double kRintShift = 6755399441055744.0;     // 1.5 * 2^52
+constexpr double kTabMaxPhase = 17592186044416.0;     // 2^44 (the shift needs |128 ph/pi| < 2^51)
 
-__device__ __forceinline__ void sincos_table(double ph, const double2 *tab, double *s, double *c) {
+__device__ __forceinline__ void sincos_table(double ph, const double2 *tab, bool wide, double *s, double *c) {
+  if (wide) {
+    const double n = rint(ph * kInv2Pi);
+    ph = fma(-n, k2PiLo, fma(-n, k2PiHi, ph));
+  }
   const double y = fma(ph, kTabPerRad, kRintShift);
   const double k = y - kRintShift;
   const double2 t = tab[(uint32_t)__double_as_longlong(y) & (kSinCosTab - 1)];

@@ -298,23 +298,32 @@ k_spectrum(const KParams p) {
   __shared__ int s_hist[kCompact ? kMaxRuns * kMaxW : 1];
   __shared__ int s_off[kCompact ? kMaxRuns * kMaxW : 1];
   __shared__ int s_cntb[kCompact ? kMaxRuns + 1 : 1];
-  // FP64 sin/cos table (sincos_table)
+  // FP64 sin/cos table (sincos_table) and the largest |omega| of the grid
   constexpr bool kTab = !kSplit && (SR_F64_TABLE != 0);
   __shared__ double2 s_sct[kTab ? kSinCosTab : 1];
-  if (kTab) {
-    for (int i = threadIdx.x; i < kSinCosTab; i += blockDim.x) {
-      double sv, cv;
-      sincospi((double)i / (kSinCosTab / 2), &sv, &cv);
-      s_sct[i] = make_double2(sv, cv);
-    }
-    __syncthreads();
-  }
+  __shared__ double s_wmax[kTab ? kMaxW : 1];
 
   const int T = blockDim.x;
   const int tid = threadIdx.x;
   const int W = T / kWave;
   const int wv = tid / kWave;
   const int lane = tid % kWave;
+  // FP64: a tile whose |tau| * om_abs_max reaches kTabMaxPhase / 2 takes the
+  // sin/cos table's wide-phase form
+  double om_abs_max = 0.0;
+  if (kTab) {
+    for (int i = tid; i < kSinCosTab; i += T) {
+      double sv, cv;
+      sincospi((double)i / (kSinCosTab / 2), &sv, &cv);
+      s_sct[i] = make_double2(sv, cv);
+    }
+    double m = 0.0;
+    for (int64_t i = tid; i < (int64_t)p.n_omega; i += T) m = fmax(m, fabs(p.omega[i]));
+    for (int o = kWave / 2; o > 0; o >>= 1) m = fmax(m, __shfl_xor(m, o));
+    if (lane == 0) s_wmax[wv] = m;
+    __syncthreads();
+    for (int w = 0; w < W; w++) om_abs_max = fmax(om_abs_max, s_wmax[w]);
+  }
   // dynamic LDS: the staged tile (s_tau[T], s_thr[T], s_rot[2][T],
   // s_amp[NA][T]), then the Kahan state, then the particle sums
   double *const s_tau = (double *)s_dyn_raw;
@@ -385,6 +394,7 @@ k_spectrum(const KParams p) {
     // range of |omega|/2pi over the block (linear grids: the run's end points)
     double w_lo = 0.0, w_hi = 0.0;
     bool rec_sortable = false;
+    bool tile_wide = false;   // FP64: the tile's phases may reach kTabMaxPhase / 2
     if (kRec) {
       const int64_t first = (int64_t)chunk * T * M;
       const int64_t last = min((int64_t)(chunk + 1) * T * M, (int64_t)p.n_omega) - 1;
@@ -646,8 +656,9 @@ k_spectrum(const KParams p) {
         }
       };
 
-      // one staged step of the direct sweeps
-      auto direct_step = [&](int j) {
+      // one staged step of the direct sweeps; WIDE (FP64): the tile's phases
+      // may reach the sin/cos table's limit
+      auto direct_step = [&](int j, auto Wc) {
         const double tau = s_tau[j];
         const double thr = kSplit ? s_thr[j] : 0.0;
         Amp a[NA];
@@ -680,7 +691,7 @@ k_spectrum(const KParams p) {
             } else {
               double sn, cn;
               if (kTab)
-                sincos_table(ph, s_sct, &sn, &cn);
+                sincos_table(ph, s_sct, decltype(Wc)::value != 0, &sn, &cn);
               else
                 sincos_reduced(ph, &sn, &cn);
               double ad[NA];
@@ -697,7 +708,7 @@ k_spectrum(const KParams p) {
       auto direct_sweep = [&](int &j, int jend) {
         if constexpr (kSplit && SR_DIRECT_GROUP) {
           const int jg = min(jend, (j + 3) & ~3);
-          for (; j < jg; j++) direct_step(j);
+          for (; j < jg; j++) direct_step(j, IC<0>());
           for (; j + 4 <= jend; j += 4) {
             const double2 t01 = *(const double2 *)&s_tau[j];
             const double2 t23 = *(const double2 *)&s_tau[j + 2];
@@ -724,8 +735,12 @@ k_spectrum(const KParams p) {
             }
           }
         }
+        if (kTab && tile_wide) {
+          for (; j < jend; j++) direct_step(j, IC<1>());
+          return;
+        }
 #pragma unroll 2
-        for (; j < jend; j++) direct_step(j);
+        for (; j < jend; j++) direct_step(j, IC<0>());
       };
       // split: Kahan fold of the segment's partial sums into the LDS S, C
       auto kahan_fold = [&]() {
@@ -748,6 +763,7 @@ k_spectrum(const KParams p) {
         const int cnt = uni((int)min((int64_t)T, n_act - t0));
         __syncthreads();  // previous tile fully consumed
         int step_on = 1;
+        int st_wide = 0;
         Amp st_amp[NA];
         double st_tau = 0.0, st_thr = 0.0;
         float st_rot[2] = {1.f, 0.f};
@@ -833,6 +849,7 @@ k_spectrum(const KParams p) {
             st_amp[5] = (Amp)(r2 * nz);
           }
           st_tau = tau;
+          if (kTab) st_wide = fabs(tau) * om_abs_max >= 0.5 * kTabMaxPhase;
           if (kSplit) {
             // |omega tau_s - omega tau_{s-1}| < pi  <=>  |omega|/2pi < (1/2)/|tau_s - tau_{s-1}|
             // (phase_{-1} = 0 at the first step); 1/0 gives +inf: always active
@@ -901,7 +918,13 @@ k_spectrum(const KParams p) {
           for (int c = 0; c < NA; c++) s_amp[c][pos] = st_amp[c];
         }
         // rec: the tile needs no masking when every step keeps the whole block
-        const bool tile_on = kRec ? (__syncthreads_and(step_on) != 0) : (__syncthreads(), false);
+        bool tile_on = false;
+        if constexpr (kRec)
+          tile_on = __syncthreads_and(step_on) != 0;
+        else if constexpr (kTab)
+          tile_wide = uni(__syncthreads_or(st_wide)) != 0;
+        else
+          __syncthreads();
 
         // ---- sweep, split at snapshot records falling inside the tile ----
         if (sort_tile) {

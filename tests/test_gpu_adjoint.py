@@ -69,6 +69,29 @@ def test_adjoint_with_active_filter(precision, tol):
     compare(per, gw, ref, ref_w, tol, f"filtered {precision}")
 
 
+def test_adjoint_f64_phases_beyond_table_range():
+    """FP64 adjoint with phases of 1.26e13..2.51e13 rad (tracks 1.25e6 m behind
+    the origin along z, so tau = t - n.x is ~1.25e6 m), on both sides of
+    2^44 = 1.76e13 rad, where the FP64 sin/cos switches from its direct table
+    reduction to a 2 pi pre-reduction; against the oracle adjoint."""
+    dt = 0.01 / 16
+    tr = []
+    for i, (g, K) in enumerate([(100.0, 1.0), (120.0, 0.8)]):
+        arrs = list(workloads.planar_undulator(g, K, 0.01, 400, dt, psi=0.3 * i))
+        arrs[2] = arrs[2] - 1.25e6
+        tr.append(arrs + [1.0 + 0.5 * i, 0])
+    axes = oracle.make_axes([(1.6e6, 3.2e6), (0.0, 0.01), (0.0, 2 * np.pi), (48, 4, 4)], "far")
+    w = 2 * np.pi * axes["omega"]
+    assert w.min() * 1.25e6 < 2.0 ** 44 < w.max() * 1.25e6
+    S0 = oracle.spectrum("far", "total", tr, axes, dt)["total"]
+    assert (S0 > 0).all()
+    rng = np.random.default_rng(5)
+    G = {"total": rng.standard_normal(S0.shape) * S0.mean()}
+    ref, ref_w = oracle.far_adjoint("total", tr, axes, dt, G)
+    per, gw = gpu_grads(tr, axes, dt, G, "total", "double")
+    compare(per, gw, ref, ref_w, 1e-10, "phases beyond 2^44")
+
+
 @pytest.mark.parametrize("groups,waves", [(1, 1), (3, 2), (7, 4)])
 def test_adjoint_decomposition_does_not_change_the_result(groups, waves):
     tr, axes, dt, S0, T, W, G = adjoint_case("total")

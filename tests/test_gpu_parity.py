@@ -183,6 +183,28 @@ def test_ragged_batch_snapshots(precision, sweep, it_range, nSnaps, slices):
     check(got, ref, precision, f"ragged it_range={it_range} nSnaps={nSnaps} slices={slices}")
 
 
+def test_f64_phases_beyond_table_range():
+    """FP64 phases of 1.26e13..2.51e13 rad (tracks starting at iteration 2e9 of
+    a global window, so t is ~1.25e6 m), on both sides of 2^44 = 1.76e13 rad, where
+    the FP64 sin/cos switches from its direct table reduction to a 2 pi
+    pre-reduction; the steps' phase increments stay below pi, so the filter
+    keeps them.  Against the oracle's libm sin/cos of the same FP64 phases."""
+    dt = 0.01 / 16
+    tr = []
+    for i, (g, K) in enumerate([(100.0, 1.0), (120.0, 0.8)]):
+        arrs = workloads.planar_undulator(g, K, 0.01, 400, dt, psi=0.3 * i)
+        tr.append(list(arrs) + [1.0 + 0.5 * i, 2_000_000_000 + 37 * i])
+    axes = oracle.make_axes([(1.6e6, 3.2e6), (0.0, 0.01), (0.0, 2 * np.pi), (48, 4, 4)], "far")
+    it_range = (2_000_000_000, 2_000_000_500)
+    w = 2 * np.pi * axes["omega"]
+    t0 = it_range[0] * dt
+    assert w.min() * t0 < 2.0 ** 44 < w.max() * t0
+    ref = oracle.spectrum("far", "total", tr, axes, dt, it_range=it_range)
+    assert (ref["total"] > 0).all()
+    got = gpu_spectrum(tr, axes, dt, precision="double", it_range=it_range)
+    check(got, ref, "double", "phases beyond 2^44")
+
+
 @pytest.mark.parametrize("precision", ["double", "float"])
 def test_uneven_csr_slices_agree(precision):
     wl = workloads.config3(scale=0.25, n_particles=40)
