@@ -29,8 +29,8 @@
 //     the tile back with broadcast LDS reads and accumulates its omegas.  The
 //     omega-independent work is paid once per (point, step) and shared by the
 //     block's T*M omegas.
-//   * Amplitude words: far split A.e_theta, A.e_phi (A is perpendicular to n);
-//     far FP64 A_x, A_y, A_z; near v1 = (b - n)/R and v2 = n/R^2.
+//   * Amplitude words: far A.e_theta, A.e_phi (A is perpendicular to n; the
+//     record rebuilds x, y, z); near v1 = (b - n)/R and v2 = n/R^2.
 //   * Sweeps:
 //       F64 direct    FP64 phase omega*tau, |dphase| < pi filter exactly as the
 //                     reference forms it, FP64 sin/cos (quadrant reduction and
@@ -184,11 +184,11 @@ constexpr size_t acc_lds_bytes(int T) {
   return (size_t)M * T * sizeof(double);
 }
 
-// amplitude words per staged step: far split A.e_theta, A.e_phi; far FP64
-// A_x, A_y, A_z; near v1 and v2 (three components each)
+// amplitude words per staged step: far A.e_theta, A.e_phi; near v1 and v2
+// (three components each)
 template <int MODE, int PREC>
 constexpr int amp_words() {
-  return MODE == SR_MODE_FAR ? (PREC == SR_PREC_SPLIT ? 2 : 3) : 6;
+  return MODE == SR_MODE_FAR ? 2 : 6;
 }
 
 // Dynamic LDS before the Kahan state: one tile of T staged steps -- tau
@@ -280,10 +280,10 @@ k_spectrum(const KParams p) {
   using Amp = typename std::conditional<PREC == SR_PREC_SPLIT, float, double>::type;
   constexpr bool kSplit = (PREC == SR_PREC_SPLIT);
   constexpr bool kRec = (SWEEP == SWEEP_REC);
-  // amplitude words per step.  Far split: A is perpendicular to n
+  // amplitude words per step.  Far: A is perpendicular to n
   // (n.A = c1 (1 - n.bbar) - c2 n.a = 0 with c1 = (n.a) c2^2), so it is kept
-  // as its two components on e_theta, e_phi.  Far FP64: A_x, A_y, A_z as the
-  // reference sums them.  Near: v1 = (b - n)/R and v2 = n/R^2.
+  // as its two components on e_theta, e_phi (the reference sums A_x, A_y, A_z;
+  // the two agree to rounding).  Near: v1 = (b - n)/R and v2 = n/R^2.
   constexpr int NA = amp_words<MODE, PREC>();
   constexpr int NC = (MODE == SR_MODE_FAR) ? 2 * NA : 6;   // real accumulators per omega
   constexpr bool kCompact = kSplit && kRec && (SR_COMPACT != 0);
@@ -480,8 +480,9 @@ k_spectrum(const KParams p) {
       // one omega's record: F are its sums (FP64), omk its 2 pi omega
       auto emit_k = [&](uint32_t slot, int k, int wk, const double *F, double omk) {
         double re[3], im[3];
-        if constexpr (MODE == SR_MODE_FAR && kSplit) {
-          // F = F_theta e_theta + F_phi e_phi
+        if constexpr (MODE == SR_MODE_FAR) {
+          // F = F_theta e_theta + F_phi e_phi (spheric, kernel_farfield.cl:442-445:
+          // the radial part n.F is 0, as n.A is)
           if (spheric) {
             re[0] = 0.0; re[1] = F[0]; re[2] = F[1];
             im[0] = 0.0; im[1] = F[2]; im[2] = F[3];
@@ -489,9 +490,6 @@ k_spectrum(const KParams p) {
             re[0] = F[0] * thx + F[1] * phx; re[1] = F[0] * thy + F[1] * phy; re[2] = F[0] * thz;
             im[0] = F[2] * thx + F[3] * phx; im[1] = F[2] * thy + F[3] * phy; im[2] = F[2] * thz;
           }
-        } else if constexpr (MODE == SR_MODE_FAR) {
-#pragma unroll
-          for (int c = 0; c < 3; c++) { re[c] = F[c]; im[c] = F[3 + c]; }
         } else {
           // near: the sums are the field, [Re F (3) | Im F (3)]
           (void)omk;
@@ -801,25 +799,12 @@ k_spectrum(const KParams p) {
               c2 = 1.0 / c2;
             }
             c1 = c1 * c2 * c2;
-            double Ax = c1 * (gx - mx) - c2 * ax;
-            double Ay = c1 * (gy - my) - c2 * ay;
-            double Az = c1 * (gz - mz) - c2 * az;
-            if constexpr (kSplit) {
-              // the two transverse components; the record rebuilds x, y, z
-              st_amp[0] = (Amp)(thx * Ax + thy * Ay + thz * Az);
-              st_amp[1] = (Amp)(phx * Ax + phy * Ay);
-            } else {
-              if (spheric) {
-                // kernel_farfield.cl:442-445
-                const double r0 = gx * Ax + gy * Ay + gz * Az;
-                const double r1 = thx * Ax + thy * Ay + thz * Az;
-                const double r2 = phx * Ax + phy * Ay + 0.0 * Az;
-                Ax = r0; Ay = r1; Az = r2;
-              }
-              st_amp[0] = (Amp)Ax;
-              st_amp[1] = (Amp)Ay;
-              st_amp[2] = (Amp)Az;
-            }
+            const double Ax = c1 * (gx - mx) - c2 * ax;
+            const double Ay = c1 * (gy - my) - c2 * ay;
+            const double Az = c1 * (gz - mz) - c2 * az;
+            // the two transverse components; the record rebuilds x, y, z
+            st_amp[0] = (Amp)(thx * Ax + thy * Ay + thz * Az);
+            st_amp[1] = (Amp)(phx * Ax + phy * Ay);
           } else {
             // kernel_nearfield.cl:67-88; omega of c1 = omega v1 is applied in the sums
             double b0x, b0y, b0z;
