@@ -33,8 +33,9 @@
 //     record rebuilds x, y, z); near v1 = (b - n)/R and v2 = n/R^2.
 //   * Sweeps:
 //       F64 direct    FP64 phase omega*tau, |dphase| < pi filter exactly as the
-//                     reference forms it, FP64 sin/cos (quadrant reduction and
-//                     Taylor polynomials), FP64 sums; 1 or 2 omegas per thread.
+//                     reference forms it, FP64 sin/cos (pi/128 reduction, an
+//                     LDS table of sin/cos(i pi/128), short polynomials), FP64
+//                     sums; 1 or 2 omegas per thread.
 //       split direct  (grids without a constant step) FP64 phase in
 //                     revolutions -> v_fract_f64 -> v_sin_f32/v_cos_f32, the
 //                     filter in its threshold form |omega|/2pi < 1/(2|dtau|),
