@@ -81,11 +81,11 @@ struct AParams {
   double *gx, *gy, *gz, *gux, *guy, *guz, *gw;
 };
 
-// TWO: split 'total', coefficients on e_theta (word 0) and e_phi (word 1)
+// TWO: 'total', coefficients on e_theta (word 0) and e_phi (word 1); the
+// fields are transverse, so H F has no part along n
 template <int PREC, bool TWO>
 __global__ void __launch_bounds__(kMaxT) k_adjoint(const AParams p) {
   constexpr bool kSplit = (PREC == SR_PREC_SPLIT);
-  static_assert(kSplit || !TWO, "two-word coefficients are split-only");
   using Acc = typename std::conditional<kSplit, float, double>::type;
   constexpr int NCW = TWO ? 2 : 3;   // coefficient words per omega: re 0..NCW-1, im NCW..2NCW-1
   constexpr int NW = 2 * NCW;
@@ -240,7 +240,7 @@ __global__ void __launch_bounds__(kMaxT) k_adjoint(const AParams p) {
         for (int c = 0; c < NCW; c++) { P[c] = 0; R[c] = 0; }
         auto acc_pair = [&](Acc sn, Acc cs, int j) {
           const Acc wr = kSplit ? (Acc)s_cwf[kSplit ? j : 0] : (Acc)s_cw[j];
-          if constexpr (TWO) {
+          if constexpr (TWO && kSplit) {
             const float4 v = *(const float4 *)&s_cf[cf_at(d, j, 0)];
             const float ar[2] = {v.x, v.y}, ai[2] = {v.z, v.w};
 #pragma unroll
@@ -464,7 +464,7 @@ APlan adjoint_plan(const sr_tracks &t, const sr_grid &g, const sr_opts &o, bool 
   int64_t groups = o.slices > 0 ? o.slices : std::max<int64_t>(1, (4096 + pl.chunk - 1) / pl.chunk);
   pl.groups = (int)std::min<int64_t>(std::min<int64_t>(groups, n_dir), 1 << 16);
   const bool split = o.precision == SR_PREC_SPLIT;
-  const size_t per_dir = (size_t)((split && o.comp == SR_COMP_TOTAL) ? 4 : 6) * g.n_omega *
+  const size_t per_dir = (size_t)((o.comp == SR_COMP_TOTAL) ? 4 : 6) * g.n_omega *
                          (split ? sizeof(float) : sizeof(double));
   const int64_t dirs_per_group = (n_dir + pl.groups - 1) / pl.groups;
   pl.dir_batch = (int)std::max<int64_t>(1, std::min<int64_t>({(int64_t)(kAdjTableBudget / per_dir), 16,
@@ -576,7 +576,9 @@ int adjoint_run(const sr_tracks *t, const sr_grid *g, const sr_opts *o, const do
     p.chunk_base = pb;
     p.chunk_n = pe - pb;
     const int64_t blocks = p.chunk_n * (int64_t)pl.groups;
-    if (o->precision == SR_PREC_F64)
+    if (o->precision == SR_PREC_F64 && o->comp == SR_COMP_TOTAL)
+      hipLaunchKernelGGL((k_adjoint<SR_PREC_F64, true>), dim3((unsigned)blocks), dim3(pl.threads), pl.lds, st, p);
+    else if (o->precision == SR_PREC_F64)
       hipLaunchKernelGGL((k_adjoint<SR_PREC_F64, false>), dim3((unsigned)blocks), dim3(pl.threads), pl.lds, st, p);
     else if (o->comp == SR_COMP_TOTAL)
       hipLaunchKernelGGL((k_adjoint<SR_PREC_SPLIT, true>), dim3((unsigned)blocks), dim3(pl.threads), pl.lds, st, p);
