@@ -250,6 +250,10 @@ struct LdsRows {
 #ifndef SR_F64_WAVES
 #define SR_F64_WAVES 4
 #endif
+// steps per iteration of the direct sweep's step loop (FP64 and the split tail)
+#ifndef SR_F64_UNROLL
+#define SR_F64_UNROLL 4
+#endif
 #ifndef SR_F64_WAVES2
 #define SR_F64_WAVES2 4
 #endif
@@ -738,7 +742,7 @@ k_spectrum(const KParams p) {
           for (; j < jend; j++) direct_step(j, IC<1>());
           return;
         }
-#pragma unroll 2
+#pragma unroll SR_F64_UNROLL
         for (; j < jend; j++) direct_step(j, IC<0>());
       };
       // split: Kahan fold of the segment's partial sums into the LDS S, C
