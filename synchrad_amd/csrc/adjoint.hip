@@ -81,6 +81,11 @@ struct AParams {
   double *gx, *gy, *gz, *gux, *guy, *guz, *gw;
 };
 
+// frequencies per iteration of the transpose sweep's direct omega loop
+#ifndef SR_ADJ_UNROLL
+#define SR_ADJ_UNROLL 4
+#endif
+
 // TWO: 'total', coefficients on e_theta (word 0) and e_phi (word 1); the
 // fields are transverse, so H F has no part along n
 template <int PREC, bool TWO>
@@ -283,6 +288,7 @@ __global__ void __launch_bounds__(kMaxT) k_adjoint(const AParams p) {
           // WIDE: a step of the wave may reach the sin/cos table's limit (a
           // compile-time flag, so the usual loop is the table form alone)
           auto om_sweep = [&](auto Wc) {
+#pragma unroll SR_ADJ_UNROLL
             for (int j = 0; j < n_om; j++) {
               const double w = s_cw[j];
               const double ph = w * tau;
