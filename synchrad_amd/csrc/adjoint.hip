@@ -85,6 +85,11 @@ struct AParams {
 #ifndef SR_ADJ_UNROLL
 #define SR_ADJ_UNROLL 4
 #endif
+// and of the split sweep's rotation loop (the recurrence is serial; unrolling
+// lets the next frequencies' table reads and sums overlap it)
+#ifndef SR_ADJ_ROT_UNROLL
+#define SR_ADJ_ROT_UNROLL 4
+#endif
 
 // TWO: 'total', coefficients on e_theta (word 0) and e_phi (word 1); the
 // fields are transverse, so H F has no part along n
@@ -273,6 +278,7 @@ __global__ void __launch_bounds__(kMaxT) k_adjoint(const AParams p) {
           const int jmax = lo;
           float er, ei, sn = 0.f, cn = 1.f;
           sincos_rev(p.domega_rev * tau, &ei, &er);
+#pragma unroll SR_ADJ_ROT_UNROLL
           for (int j = 0; j < jmax; j++) {
             if ((j & 31) == 0) {
               sincos_rev(s_cw[j] * tau, &sn, &cn);
