@@ -290,6 +290,19 @@ def uniform_step(w: np.ndarray, rtol: float = 1e-14) -> float:
     return float(step) if dev <= rtol * np.max(np.abs(w)) else 0.0
 
 
+def _launch_stream(stream, device):
+    """The stream a library call is ordered on: ``stream``, or the current
+    one.  A stream other than the current one first waits for the current
+    stream's work, which made or cleared the call's inputs, outputs and
+    workspace (torch allocates and fills them there); the caller orders its
+    own later use of the outputs after ``stream``."""
+    cur = torch.cuda.current_stream(device)
+    if stream is None or stream == cur:
+        return cur
+    stream.wait_stream(cur)
+    return stream
+
+
 SWEEPS = {"auto": _abi.SR_SWEEP_AUTO, "direct": _abi.SR_SWEEP_DIRECT,
           "recurrence": _abi.SR_SWEEP_RECURRENCE}
 
@@ -368,7 +381,7 @@ def spectrum(batch: DeviceBatch, grid: DeviceGrid, comp: str = "total", precisio
         else:
             ws = torch.empty(nbytes.value, dtype=torch.uint8, device=grid.device)
     out_ptrs = (ctypes.c_void_p * n_out)(*[o.data_ptr() for o in outs])
-    st = stream if stream is not None else torch.cuda.current_stream(grid.device)
+    st = _launch_stream(stream, grid.device)
     _abi.check(L.sr_spectrum_batch(ctypes.byref(tr), ctypes.byref(gr), ctypes.byref(opts), out_ptrs,
                                    None if ws is None else ctypes.c_void_p(ws.data_ptr()),
                                    nbytes.value, ctypes.c_void_p(st.cuda_stream)))
@@ -494,7 +507,7 @@ def far_fields(batch: DeviceBatch, grid: DeviceGrid, comp: str = "total", precis
                                                ctypes.byref(nbytes)))
     ws = torch.empty(max(int(nbytes.value), 1), dtype=torch.uint8, device=dev)
     optr = (ctypes.c_void_p * n_out)(*[o.data_ptr() for o in outs])
-    st = stream if stream is not None else torch.cuda.current_stream(dev)
+    st = _launch_stream(stream, dev)
     _abi.check(L.sr_far_fields_batch(ctypes.byref(tr), ctypes.byref(gr), ctypes.byref(opts),
                                      ctypes.c_void_p(fields.data_ptr()), optr, ctypes.c_void_p(ws.data_ptr()),
                                      nbytes.value, ctypes.c_void_p(st.cuda_stream)))
@@ -539,7 +552,7 @@ def far_adjoint(batch: DeviceBatch, grid: DeviceGrid, dS: List[torch.Tensor], co
     _abi.check(size_fn(ctypes.byref(tr), ctypes.byref(gr), ctypes.byref(opts), ctypes.byref(nbytes)))
     ws = torch.empty(max(int(nbytes.value), 1), dtype=torch.uint8, device=dev)
     dptr = (ctypes.c_void_p * n_in)(*[d.data_ptr() for d in dS])
-    st = stream if stream is not None else torch.cuda.current_stream(dev)
+    st = _launch_stream(stream, dev)
     tail = [*[g.data_ptr() for g in grads], ctypes.c_void_p(ws.data_ptr()), nbytes.value,
             ctypes.c_void_p(st.cuda_stream)]
     if fields is None:

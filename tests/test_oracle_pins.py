@@ -275,9 +275,78 @@ def pin_snapshots_are_truncations(library=None):
     return bool(np.array_equal(two[0], cut[0]) and np.array_equal(two[1], full[0]))
 
 
+def gaussian_form_factor_quadrature(omega, sigma, n_nodes=4001):
+    """The characteristic function of a Gaussian charge density of rms size
+    sigma at wavenumber k = 2 pi omega, by trapezoidal quadrature of
+    rho(x) cos(k x) over +-12 sigma (no closed form used)."""
+    x = np.linspace(-12.0 * sigma, 12.0 * sigma, n_nodes)
+    rho = np.exp(-0.5 * (x / sigma) ** 2)
+    rho /= np.trapezoid(rho, x) if hasattr(np, "trapezoid") else np.trapz(rho, x)
+    k = 2.0 * np.pi * np.asarray(omega)[:, None]
+    integrand = rho[None, :] * np.cos(k * x[None, :])
+    return np.trapezoid(integrand, x, axis=1) if hasattr(np, "trapezoid") else np.trapz(integrand, x, axis=1)
+
+
+def pin_form_factor_coherent(library=None):
+    """Coherent far field of one w = 1 particle with a particle size sigma:
+    every cartesian_complex component is the sigma = 0 one times the Gaussian
+    density's characteristic function at k = 2 pi omega (by quadrature), the
+    reference's FormFactor (calc.py:475-480 applied at kernel_farfield.cl:
+    324-325); spheric_complex takes the argument and ignores it
+    (kernel_farfield.cl:576-577), so it does not change with sigma."""
+    wl = workloads.config1(scale=0.0625)
+    tr = wl.track_list()
+    ax = wl.axes
+    dt = wl.batch.dt
+    om = ax["omega"]
+    sigma = 0.6 / (2 * np.pi * om.mean())          # form factor between ~0.6 and ~0.9 over the grid
+    ff = gaussian_form_factor_quadrature(om, sigma)
+    if not (ff.min() < 0.9 and ff.max() < 0.99):
+        return False
+    c0 = oracle.spectrum("far", "cartesian_complex", tr, ax, dt, library=library)
+    cs = oracle.spectrum("far", "cartesian_complex", tr, ax, dt, sigma_particle=sigma, library=library)
+    scale = max(np.abs(v).max() for v in c0.values())
+    ok = True
+    for k in c0:
+        want = c0[k] * ff[None, :, None, None]
+        ok &= np.allclose(cs[k], want, rtol=1e-9, atol=1e-12 * scale)
+    s0 = oracle.spectrum("far", "spheric_complex", tr, ax, dt, library=library)
+    ss = oracle.spectrum("far", "spheric_complex", tr, ax, dt, sigma_particle=sigma, library=library)
+    ok &= all(np.array_equal(ss[k], s0[k]) for k in s0)
+    # the coherent spheric components square to the incoherent spheric ones
+    sph = oracle.spectrum("far", "spheric", tr, ax, dt, library=library)
+    sscale = max(np.abs(v).max() for v in sph.values())
+    for c in ("r", "theta", "phi"):
+        ok &= np.allclose(s0[c + "re"] ** 2 + s0[c + "im"] ** 2, sph[c], rtol=1e-9, atol=1e-12 * sscale)
+    return bool(ok)
+
+
+def pin_near_coherent(library=None):
+    """Near field, one w = 1 particle: the coherent components square to the
+    incoherent cartesian ones (the records add sqrt(w) dt F and w dt^2 |F|^2,
+    kernel_nearfield.cl:197-211, 312-323), and the particle size does not
+    enter the near field (its kernel never reads FormFactor)."""
+    wl = workloads.config4(scale=0.0625, n_particles=1)
+    tr = wl.track_list()
+    tr[0][6] = 1.0
+    ax = wl.axes
+    dt = wl.batch.dt
+    L = wl.L_screen
+    car = oracle.spectrum("near", "cartesian", tr, ax, dt, L_screen=L, library=library)
+    cc = oracle.spectrum("near", "cartesian_complex", tr, ax, dt, L_screen=L, library=library)
+    cs = oracle.spectrum("near", "cartesian_complex", tr, ax, dt, L_screen=L, sigma_particle=1e-6,
+                         library=library)
+    scale = max(np.abs(v).max() for v in car.values())
+    ok = all(np.array_equal(cs[k], cc[k]) for k in cc)
+    for c in "xyz":
+        ok &= np.allclose(cc[c + "re"] ** 2 + cc[c + "im"] ** 2, car[c], rtol=1e-9, atol=1e-12 * scale)
+    ok &= scale > 0
+    return bool(ok)
+
+
 ALL_PINS = [pin_single_interval, pin_filter_two_intervals, pin_uniform_motion, pin_undulator_line,
             pin_near_to_far, pin_near_potentials, pin_spheric_and_complex,
-            pin_snapshots_are_truncations]
+            pin_snapshots_are_truncations, pin_form_factor_coherent, pin_near_coherent]
 
 
 @pytest.mark.parametrize("pin", ALL_PINS, ids=lambda f: f.__name__)
@@ -388,6 +457,13 @@ MUTANTS = {
     "spheric_theta_sign": ("const v3 thVec = mk(cosTheta[iTheta] * cosPhi[iPhi], cosTheta[iTheta] * sinPhi[iPhi],\n                        -sinTheta[iTheta]);",
                            "const v3 thVec = mk(cosTheta[iTheta] * cosPhi[iPhi], cosTheta[iTheta] * sinPhi[iPhi],\n                        sinTheta[iTheta]);"),
     # first occurrence: the far-field kernel's record test
+    # the form factor also applied to spheric_complex (two sites)
+    "form_factor_on_spheric": [("const double ffLocal = (comp == OR_CARTESIAN_COMPLEX) ? formFactor[iOmega] : 1.0;",
+                                "const double ffLocal = (comp >= OR_CARTESIAN_COMPLEX) ? formFactor[iOmega] : 1.0;"),
+                               ("if (comp == OR_CARTESIAN_COMPLEX) {\n            /* (amplitude*cos)*FormFactor",
+                                "if (comp >= OR_CARTESIAN_COMPLEX) {\n            /* (amplitude*cos)*FormFactor")],
+    "form_factor_dropped": ("const double ffLocal = (comp == OR_CARTESIAN_COMPLEX) ? formFactor[iOmega] : 1.0;",
+                            "const double ffLocal = 1.0;"),
     "snapshot_off_by_one": ("if (iSnap < nSnaps && it_glob + 2 == itSnaps[iSnap]) {",
                             "if (iSnap < nSnaps && it_glob + 1 == itSnaps[iSnap]) {"),
 }
@@ -397,10 +473,12 @@ MUTANTS = {
 @pytest.mark.parametrize("name", sorted(MUTANTS))
 def test_planted_mistakes_are_caught(name, tmp_path):
     src = open(os.path.join(os.path.dirname(oracle.__file__), "oracle.c")).read()
-    old, new = MUTANTS[name]
-    assert src.count(old) >= 1, f"mutation site for {name} not found"
+    sites = MUTANTS[name] if isinstance(MUTANTS[name], list) else [MUTANTS[name]]
+    for old, new in sites:
+        assert src.count(old) >= 1, f"mutation site for {name} not found"
+        src = src.replace(old, new, 1)
     mut = tmp_path / "oracle_mut.c"
-    mut.write_text(src.replace(old, new, 1))
+    mut.write_text(src)
     so = tmp_path / "liboracle_mut.so"
     subprocess.run(["gcc", "-O2", "-fPIC", "-fopenmp", "-ffp-contract=off", "-shared", "-o", str(so),
                     str(mut), "-lm"], check=True)
