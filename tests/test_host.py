@@ -260,3 +260,97 @@ def test_autotuner_leaves_large_far_launches_to_the_library_plan():
     assert case("far", 256 * 32 * 32, 1599)                # one undulator electron, 4.2e8
     assert case("near", 128 * 64 * 64, 500 * 3999)         # near field: not reordered
     assert case("far", 512 * 64 * 64, 100) and not case("far", 512 * 64 * 64, 500_000)
+
+
+def _polar_spec(mode, comp="total", n=(6, 9, 8), seed=5):
+    rng = np.random.default_rng(seed)
+    omega = np.linspace(1.0, 4.0, n[0])
+    ax1 = np.linspace(0.0, 0.05, n[1])
+    phi = np.linspace(0, 2 * np.pi, n[2], endpoint=False)
+    keys = engine.COMP_KEYS[comp]
+    rad = {k: rng.random((2,) + n) for k in keys}
+    if comp == "total":
+        rad["total"][:, :, 0, :] = rad["total"][:, :, 0, :1]   # one value at r = 0 for every phi
+    Args = {"mode": mode, "comp": comp, "omega": omega, "phi": phi, "dph": phi[1] - phi[0],
+            "dw": np.abs(np.diff(omega)), ("theta" if mode == "far" else "radius"): ax1}
+    return _Spec(rad, Args)
+
+
+def test_radial_integral_exact_for_constant_spectra():
+    """Near field: the screen integral of a constant c is c * pi * r_max^2
+    (trapezoid rule of the linear r c, times 2 pi from the phi sum)."""
+    s = _polar_spec("near")
+    s.Data["radiation"]["total"][:] = 3.0
+    pref = alpha_fs * np.pi / 4 / (2 * np.pi) ** 2
+    es = s.get_energy_spectrum()
+    np.testing.assert_allclose(es, pref * 3.0 * np.pi * 0.05 ** 2, rtol=1e-13)
+    assert s.get_energy() == pytest.approx(pref * 3.0 * np.pi * 0.05 ** 2 * 3.0, rel=1e-13)
+
+
+def test_quadrature_weights_are_the_trapezoid_rule():
+    from synchrad_amd import utils
+    rng = np.random.default_rng(1)
+    x = np.sort(rng.random(17))
+    f = rng.random(17)
+    trap = getattr(np, "trapezoid", None) or np.trapz
+    assert utils.trapezoid_weights(x) @ f == pytest.approx(trap(f, x), rel=1e-14)
+    assert utils.trapezoid_weights(x[::-1]) @ f[::-1] == pytest.approx(-trap(f, x), rel=1e-14)
+    assert utils.trapezoid_weights(x[:1]).tolist() == [0.0]
+    # the polar rule on theta midpoints of midpoint averages times sin
+    th = np.sort(rng.random(9)) * 0.1
+    mid = 0.5 * (th[1:] + th[:-1])
+    want = trap(0.5 * (f[1:9] + f[:8]) * np.sin(mid), mid)
+    assert utils.polar_angle_weights(th) @ f[:9] == pytest.approx(want, rel=1e-13)
+
+
+def test_spot_frequency_node_rule():
+    """get_spot(k0) reads the first node not below k0, or the next one when
+    that is closer (synchrad/utils.py:119-124)."""
+    from synchrad_amd import utils
+    om = np.array([1.0, 2.0, 3.0, 4.0])
+    assert utils.nearest_node_rule(om, 1.9) == 1       # first node >= k0
+    assert utils.nearest_node_rule(om, 2.0) == 1
+    assert utils.nearest_node_rule(om, 0.2) == 0
+    assert utils.nearest_node_rule(om, 2.7) == 2       # node 2 (3.0) is above and nearest
+    assert utils.nearest_node_rule(np.array([1.0, 1.1, 5.0]), 0.5) == 0
+    assert utils.nearest_node_rule(np.array([1.0, 1.1, 5.0]), 1.05) == 1
+    assert utils.nearest_node_rule(om, 9.0) == 3
+    s = _polar_spec("far")
+    full = s.get_full_spectrum()
+    # omega = 1, 1.6, 2.2, 2.8, 3.4, 4: k0 = 2.45 reads 2.8 (the first node above
+    # it), although 2.2 lies closer -- the reference compares 2.8 with 3.4 only
+    assert np.array_equal(s.get_spot(k0=2.45), full[3])
+    trap = getattr(np, "trapezoid", None) or np.trapz
+    np.testing.assert_allclose(s.get_spot(), trap(full, s.Args["omega"], axis=0), rtol=1e-14)
+
+
+@pytest.mark.parametrize("mode", ["far", "near"])
+def test_spot_cartesian_equals_scipy_griddata(mode):
+    """The Cartesian resampling is griddata's linear interpolation over the
+    polar nodes, zero outside their hull (synchrad/utils.py:140-151)."""
+    from scipy.interpolate import griddata
+    s = _polar_spec(mode)
+    spot = s.get_spot()
+    r = s.Args["theta" if mode == "far" else "radius"]
+    ph, th = np.meshgrid(s.Args["phi"], r)
+    pts = ((th * np.cos(ph)).ravel(), (th * np.sin(ph)).ravel())
+    m = 0.7 * r.max()
+    mesh = np.mgrid[-m:m:40j, -m:m:30j]
+    want = griddata(pts, spot.ravel(), (mesh[0].ravel(), mesh[1].ravel()), fill_value=0.0,
+                    method="linear").reshape(mesh[0].shape)
+    got, ext = s.get_spot_cartesian(th_part=0.7, bins=(40, 30))
+    assert got.shape == (40, 30) and ext.tolist() == [-m, m, -m, m]
+    np.testing.assert_allclose(got, want, rtol=1e-13, atol=1e-15 * np.abs(want).max())
+
+
+def test_coherent_components():
+    """Coherent runs: 'total' sums the squares of the six outputs, a named
+    component is the complex sum re + i im."""
+    s = _polar_spec("far", comp="cartesian_complex")
+    rad = s.Data["radiation"]
+    pref = alpha_fs / (4 * np.pi ** 2)
+    tot = s.get_full_spectrum()
+    np.testing.assert_allclose(tot, pref * sum(rad[k][-1] ** 2 for k in rad), rtol=1e-14)
+    y = s.get_full_spectrum(comp="y", iteration=0)
+    assert np.iscomplexobj(y)
+    np.testing.assert_allclose(y, pref * (rad["yre"][0] + 1j * rad["yim"][0]), rtol=1e-14)
