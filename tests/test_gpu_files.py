@@ -69,3 +69,45 @@ def test_track_file_to_spectrum_file(tmp_path, global_range, dtype, tol):
     assert back.total_weight == pytest.approx(calc.total_weight)
     # the analysis helpers work on the reloaded file
     assert np.isclose(back.get_energy(lambda0_um=1.0), calc.get_energy(lambda0_um=1.0), rtol=1e-12)
+
+
+@pytest.mark.skipif(not _have_h5, reason="no HDF5 C library")
+@pytest.mark.parametrize("dtype,tol", [("double", 1e-10), ("float", 1e-4)])
+def test_nan_gapped_histories_to_spectrum(tmp_path, dtype, tol):
+    """Undulator histories with NaN gaps (particles entering and leaving the
+    tracker) cut into pieces and written as a track file by the converter;
+    the file's spectrum (global misc/it_range, per-piece it_start) matches the
+    oracle on the same pieces, and the CSR pieces give the same spectrum
+    directly."""
+    from synchrad.calc import SynchRad
+    from synchrad_amd import converters, engine
+    rng = np.random.default_rng(31)
+    n_p, n_it, dt = 4, 900, 0.01 / 24
+    hist = np.zeros((7, n_p, n_it))
+    for p in range(n_p):
+        arrs = workloads.planar_undulator(rng.uniform(40, 80), rng.uniform(0.5, 1.5), 0.01, n_it, dt,
+                                          psi=rng.uniform(0, 6.28))
+        hist[:6, p] = np.stack(arrs)
+        hist[6, p] = rng.uniform(0.5, 1.5)
+    hist[6, 0, 100:130] = np.nan
+    hist[6, 1, :250] = np.nan
+    hist[6, 2, 600:] = np.nan
+    hist[6, 3, 40:45] = np.nan
+    path = str(tmp_path / "pieces.h5")
+    pieces = converters.tracks_from_arrays(*hist, cdt=dt, fname=path, shortest_track=8)
+    assert pieces.n_tracks == 7
+    axes = oracle.make_axes([(100.0, 4e4), (0.0, 0.03), (0.0, 2 * np.pi), (40, 6, 5)], "far")
+    grid = [(100.0, 4e4), (0.0, 0.03), (0.0, 2 * np.pi), (40, 6, 5)]
+    calc = SynchRad({"grid": grid, "dtype": dtype})
+    calc.calculate_spectrum(file_tracks=path, comp="total", verbose=False)
+    got = calc.Data["radiation"]["total"]
+    it_range = tuple(pieces.it_range())
+    ref = oracle.spectrum("far", "total", pieces.track_list(), axes, dt, it_range=it_range)["total"]
+    r = np.linalg.norm(got - ref) / np.linalg.norm(ref)
+    print(f"converted pieces {dtype}: rel-L2 vs oracle {r:.3e}", flush=True)
+    assert r <= tol
+    ax1 = axes["theta"]
+    dgrid = engine.DeviceGrid("far", axes["omega"], ax1, axes["phi"])
+    direct = engine.spectrum(engine.DeviceBatch(pieces.host_batch(dt, it_range=it_range)), dgrid, comp="total",
+                             precision=dtype)[0]
+    assert np.array_equal(direct.cpu().numpy().swapaxes(-1, -3), got)
