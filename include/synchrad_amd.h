@@ -160,6 +160,25 @@ int sr_spectrum_batch(const sr_tracks *tracks, const sr_grid *grid, const sr_opt
                       void *stream);
 
 /*
+ * sr_spectrum_batch without a workspace argument and without any allocation:
+ * no counterpart in the reference.  It is for callers that cannot allocate
+ * device memory (code under HIP graph capture, tight serving loops, harnesses
+ * that forbid hipMalloc).  The tile kernels compute the direction-independent
+ * step data themselves, and the particle slices' partial spectra and the tile
+ * launch order live in a static device array of SR_NOALLOC_SCRATCH_BYTES per
+ * GPU, held from the library's load to the end of the process.  The plan is
+ * sr_spectrum_batch's; with opts.slices = 0 the slice count is halved until
+ * its partial spectra fit the array, and a request that still does not fit
+ * (an explicit opts.slices) fails with SR_ERR_WORKSPACE.  Results equal
+ * sr_spectrum_batch's with the same slice count.  Every call on one GPU uses
+ * that GPU's one array: calls that overlap in time on one GPU (concurrent
+ * streams, or several host threads) race and must be serialised by the caller.
+ */
+#define SR_NOALLOC_SCRATCH_BYTES ((size_t)64 << 20)
+int sr_spectrum_batch_noalloc(const sr_tracks *tracks, const sr_grid *grid, const sr_opts *opts,
+                              double *const *outs, void *stream);
+
+/*
  * Adjoint of the far-field spectrum (comp SR_COMP_TOTAL or SR_COMP_CARTESIAN,
  * one snapshot).  No counterpart in the reference, which has no gradients.
  * Given dS = dL/dS (1 or 3 device arrays laid out like the spectra) it ADDS

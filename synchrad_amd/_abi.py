@@ -83,8 +83,10 @@ EXPORTS = (
     "sr_spectrum_batch", "sr_far_total", "sr_far_cartesian", "sr_near_total",
     "sr_near_cartesian", "sr_far_adjoint_workspace_bytes", "sr_far_adjoint_batch",
     "sr_far_fields_workspace_bytes", "sr_far_fields_batch", "sr_far_adjoint_fields_workspace_bytes",
-    "sr_far_adjoint_from_fields",
+    "sr_far_adjoint_from_fields", "sr_spectrum_batch_noalloc",
 )
+
+SR_NOALLOC_SCRATCH_BYTES = 64 << 20
 
 _lib = None
 
@@ -116,6 +118,9 @@ def load(path: str = LIB):
     L.sr_spectrum_batch.argtypes = [ctypes.POINTER(SrTracks), ctypes.POINTER(SrGrid),
                                     ctypes.POINTER(SrOpts), ctypes.POINTER(_vp), _vp,
                                     ctypes.c_size_t, _vp]
+    L.sr_spectrum_batch_noalloc.restype = ctypes.c_int
+    L.sr_spectrum_batch_noalloc.argtypes = [ctypes.POINTER(SrTracks), ctypes.POINTER(SrGrid),
+                                            ctypes.POINTER(SrOpts), ctypes.POINTER(_vp), _vp]
     track_args = [_vp] * 6 + [_f64, _u32, _u32, _u32]
     tail = [_u32] * 3 + [_f64, _u32, _vp, _i32, _vp]
     L.sr_far_total.restype = ctypes.c_int

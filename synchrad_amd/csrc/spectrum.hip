@@ -67,6 +67,7 @@
 #include <math.h>
 
 #include <algorithm>
+#include <mutex>
 #include <type_traits>
 
 #include "synchrad_amd.h"
@@ -1104,6 +1105,44 @@ __global__ void k_fill_zero(double *buf, int64_t n) {
   if (i < n) buf[i] = 0.0;
 }
 
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// scratch of the no-allocation entry point
+// ---------------------------------------------------------------------------
+// sr_spectrum_batch_noalloc keeps its workspace (the particle slices' partial
+// spectra and the tile launch order) in this array instead of a caller's
+// buffer.  Each GPU holds its own copy from the library's load to the end of
+// the process.  On the host the name is only a stand-in: its device address
+// on the current GPU comes from hipGetSymbolAddress, once per GPU.
+constexpr size_t kNoAllocScratchBytes = SR_NOALLOC_SCRATCH_BYTES;
+static __device__ __align__(256) unsigned char scratch[kNoAllocScratchBytes];
+
+namespace {
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+void *g_scratch_ptr[kMaxDevices] = {};
+
+// device address of `scratch` on the current GPU, looked up on the first call
+// made with that GPU current
+int noalloc_scratch(void **out) {
+  int dev = -1;
+  hipError_t e = hipGetDevice(&dev);
+  if (e != hipSuccess) return fail(SR_ERR_HIP, "hipGetDevice: %s", hipGetErrorString(e));
+  if (dev < 0 || dev >= kMaxDevices) return fail(SR_ERR_UNSUPPORTED, "device %d: at most %d devices", dev, kMaxDevices);
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_ptr[dev]) {
+    void *p = nullptr;
+    e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p)
+      return fail(SR_ERR_HIP, "hipGetSymbolAddress(scratch) on device %d: %s", dev, hipGetErrorString(e));
+    g_scratch_ptr[dev] = p;
+  }
+  *out = g_scratch_ptr[dev];
+  return SR_OK;
+}
+
 // ---------------------------------------------------------------------------
 // host side
 // ---------------------------------------------------------------------------
@@ -1343,11 +1382,12 @@ size_t order_bytes(const Plan &pl, const sr_grid &g, const sr_opts &o, int64_t n
 }
 
 // want_pre: compute the step data into the workspace (batch calls); the
-// single-particle entry points have no workspace and compute it in the stage
-// fields_order: bytes of the tile order for a fields launch (its workspace is
-// only the order; batch calls size theirs from the plan)
+// single-particle entry points and the no-allocation batch call compute it in
+// the stage instead.  need_order: bytes of the tile launch order (0: index
+// order), order_bytes() for batch calls.  Workspace layout: the slices'
+// partial spectra, the step data, the tile costs and order.
 int run(KParams &kp, const Plan &pl, const sr_grid &g, const sr_opts &o, void *ws, size_t ws_bytes,
-        hipStream_t st, bool want_pre = false, size_t fields_order = 0) {
+        hipStream_t st, bool want_pre = false, size_t need_order = 0) {
   const int threads = kWave * pl.waves;
   const int64_t run = (int64_t)threads * pl.M;
   kp.n_chunks = (uint32_t)((g.n_omega + run - 1) / run);
@@ -1357,7 +1397,6 @@ int run(KParams &kp, const Plan &pl, const sr_grid &g, const sr_opts &o, void *w
   // per-particle outputs are disjoint: slices write them in place
   const size_t need = kp.part_stride ? 0 : scratch_bytes(pl, kp.n_out, kp.n_grid, kp.n_snaps);
   const size_t need_pre = want_pre ? pre_bytes(g.mode, kp.n_points) : 0;
-  const size_t need_order = want_pre ? order_bytes(pl, g, o, kp.n_points) : fields_order;
   hipError_t e;
   if (need + need_pre + need_order && (!ws || ws_bytes < need + need_pre + need_order))
     return fail(SR_ERR_WORKSPACE, "workspace of %zu bytes needed, %zu given", need + need_pre + need_order,
@@ -1587,8 +1626,55 @@ int sr_spectrum_batch(const sr_tracks *t, const sr_grid *g, const sr_opts *o, do
   fill_grid(kp, *g, *o, n_out, outs);
   const Plan pl = make_plan(*g, *o, t->n_particles, t->n_snaps, t->n_points);
   slice_particles(kp, pl.slices, t->n_particles, t->offsets_host);
-  return run(kp, pl, *g, *o, workspace, workspace_bytes, (hipStream_t)stream, true);
+  return run(kp, pl, *g, *o, workspace, workspace_bytes, (hipStream_t)stream, true,
+             order_bytes(pl, *g, *o, t->n_points));
 }
+
+int sr_spectrum_batch_noalloc(const sr_tracks *t, const sr_grid *g, const sr_opts *o, double *const *outs,
+                              void *stream) {
+  int rc;
+  if ((rc = check_tracks(t)) || (rc = check_grid(g)) || (rc = check_opts(g, o))) return rc;
+  if (t->n_particles == 0) return SR_OK;
+  const int n_out = outputs_for(g->mode, o->comp);
+  if (!outs) return fail(SR_ERR_INVALID, "outs is NULL");
+  for (int c = 0; c < n_out; c++)
+    if (!outs[c]) return fail(SR_ERR_INVALID, "output %d is NULL", c);
+  void *scr = nullptr;
+  if ((rc = noalloc_scratch(&scr))) return rc;
+  KParams kp;
+  memset(&kp, 0, sizeof(kp));
+  kp.offsets = t->offsets;
+  kp.x = t->x; kp.y = t->y; kp.z = t->z; kp.ux = t->ux; kp.uy = t->uy; kp.uz = t->uz;
+  kp.weight = t->weight;
+  kp.it_start = t->it_start;
+  kp.it_end = t->it_end;
+  kp.it_snaps = t->it_snaps;
+  kp.n_snaps = t->n_snaps;
+  kp.snaps_stride = t->snaps_stride;
+  kp.dt = t->dt;
+  kp.n_points = t->n_points;
+  fill_grid(kp, *g, *o, n_out, outs);
+  Plan pl = make_plan(*g, *o, t->n_particles, t->n_snaps, t->n_points);
+  // the same plan as sr_spectrum_batch; a tuned slice count is halved until
+  // the slices' partial spectra and the tile order fit the scratch array
+  auto need = [&](const Plan &p) {
+    return scratch_bytes(p, n_out, kp.n_grid, t->n_snaps) + order_bytes(p, *g, *o, t->n_points);
+  };
+  while (o->slices == 0 && pl.slices > 1 && need(pl) > kNoAllocScratchBytes) {
+    pl.blocks /= pl.slices;
+    pl.slices /= 2;
+    pl.blocks *= pl.slices;
+  }
+  if (need(pl) > kNoAllocScratchBytes)
+    return fail(SR_ERR_WORKSPACE,
+                "this request needs %zu bytes of scratch (%d particle slices); the no-allocation entry "
+                "point holds %zu: ask for fewer slices (opts.slices) or use sr_spectrum_batch",
+                need(pl), pl.slices, kNoAllocScratchBytes);
+  slice_particles(kp, pl.slices, t->n_particles, t->offsets_host);
+  return run(kp, pl, *g, *o, scr, kNoAllocScratchBytes, (hipStream_t)stream, false,
+             order_bytes(pl, *g, *o, t->n_points));
+}
+
 
 int sr_far_total(double *spectrum, const double *x, const double *y, const double *z,
                  const double *ux, const double *uy, const double *uz, double wp, uint32_t itStart,
