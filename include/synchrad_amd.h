@@ -23,7 +23,7 @@
 extern "C" {
 #endif
 
-#define SR_ABI_VERSION 2
+#define SR_ABI_VERSION 3
 
 typedef enum sr_status {
   SR_OK = 0,
@@ -99,7 +99,13 @@ typedef struct sr_tracks {
 /*
  * Spectral grid, as SynchRad._init_data uploads it (synchrad/calc.py:494-512).
  * Far: omega (already multiplied by 2*pi), sin/cos of theta and phi.
- * Near: omega, radius, sin/cos of phi and the screen distance.
+ * Near: omega, radius, sin/cos of phi and the screen distance; the observer
+ * of (radius i, phi j) is (radius[i] cos phi_j, radius[i] sin phi_j, l_screen)
+ * (kernel_nearfield.cl:40-42).  A Cartesian screen (no counterpart in the
+ * reference, whose screen is polar) sets screen_y: the observer of (i, j) is
+ * then (radius[i], screen_y[j], l_screen) -- radius holds the n_theta x
+ * positions, screen_y the n_phi y positions -- and sin_phi, cos_phi are not
+ * read.
  */
 typedef struct sr_grid {
   int32_t mode;             /* SR_MODE_FAR / SR_MODE_NEAR */
@@ -114,6 +120,7 @@ typedef struct sr_grid {
   const double *cos_phi;    /* device [n_phi] */
   double l_screen;          /* near: distance to the screen */
   double omega_step;        /* omega[i+1] - omega[i] if constant (linear grids), else 0 */
+  const double *screen_y;   /* near: device [n_phi] y positions of a Cartesian screen, or NULL */
 } sr_grid;
 
 typedef struct sr_opts {
@@ -158,6 +165,43 @@ int sr_workspace_bytes(const sr_tracks *tracks, const sr_grid *grid, const sr_op
 int sr_spectrum_batch(const sr_tracks *tracks, const sr_grid *grid, const sr_opts *opts,
                       double *const *outs, void *workspace, size_t workspace_bytes,
                       void *stream);
+
+/*
+ * The plan sr_spectrum_batch resolves for a request: sweep, omegas per lane,
+ * waves per block, particle slices, tiles, and whether the tiles are launched
+ * heaviest first.  No counterpart in the reference (whose work-group size is
+ * fixed, synchrad/calc.py:633-646).
+ */
+typedef struct sr_plan_info {
+  int32_t sweep;            /* SR_SWEEP_DIRECT or SR_SWEEP_RECURRENCE */
+  int32_t omega_per_lane;
+  int32_t waves_per_block;
+  int32_t slices;
+  int64_t tiles;
+  int32_t ordered;          /* 1: heaviest tiles first */
+} sr_plan_info;
+int sr_plan(const sr_tracks *tracks, const sr_grid *grid, const sr_opts *opts, sr_plan_info *info);
+
+/*
+ * Launch-shape autotuning.  Times sr_spectrum_batch on the given request once
+ * per candidate -- each candidate's plan fields (sweep, omega_per_lane,
+ * waves_per_block, slices, persistent, tile_order) over base's comp,
+ * precision and form_factor -- with `warmup` untimed runs and `reps` timed
+ * ones, HIP events on `stream`; ms[i] is candidate i's mean time (INFINITY for a
+ * plan the request cannot run or whose workspace does not fit) and *best the
+ * fastest.  The caller chooses the request: the real one, or a representative
+ * sample of it (the Python host samples large launches as full-length tracks
+ * on a subset of the directions).  `outs` receive the sum of all runs (scratch
+ * to the caller); the workspace must hold sr_autotune_workspace_bytes.  Unlike
+ * every other entry point this one synchronises the stream (to read the
+ * timers).  No counterpart in the reference.
+ */
+int sr_autotune_workspace_bytes(const sr_tracks *tracks, const sr_grid *grid, const sr_opts *base,
+                                const sr_opts *candidates, int n_candidates, size_t *bytes);
+int sr_autotune(const sr_tracks *tracks, const sr_grid *grid, const sr_opts *base,
+                const sr_opts *candidates, int n_candidates, double *const *outs,
+                void *workspace, size_t workspace_bytes, int warmup, int reps, void *stream,
+                float *ms, int *best);
 
 /*
  * sr_spectrum_batch without a workspace argument and without any allocation:

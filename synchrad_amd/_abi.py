@@ -10,7 +10,7 @@ import os
 
 from ._build import LIB
 
-SR_ABI_VERSION = 2
+SR_ABI_VERSION = 3
 
 SR_OK = 0
 SR_ERR_INVALID = -1
@@ -67,7 +67,13 @@ class SrGrid(ctypes.Structure):
         ("mode", _i32), ("n_omega", _u32), ("n_theta", _u32), ("n_phi", _u32),
         ("omega", _vp), ("sin_theta", _vp), ("cos_theta", _vp), ("radius", _vp),
         ("sin_phi", _vp), ("cos_phi", _vp), ("l_screen", _f64), ("omega_step", _f64),
+        ("screen_y", _vp),
     ]
+
+
+class SrPlanInfo(ctypes.Structure):
+    _fields_ = [("sweep", _i32), ("omega_per_lane", _i32), ("waves_per_block", _i32), ("slices", _i32),
+                ("tiles", _i64), ("ordered", _i32)]
 
 
 class SrOpts(ctypes.Structure):
@@ -83,7 +89,8 @@ EXPORTS = (
     "sr_spectrum_batch", "sr_far_total", "sr_far_cartesian", "sr_near_total",
     "sr_near_cartesian", "sr_far_adjoint_workspace_bytes", "sr_far_adjoint_batch",
     "sr_far_fields_workspace_bytes", "sr_far_fields_batch", "sr_far_adjoint_fields_workspace_bytes",
-    "sr_far_adjoint_from_fields", "sr_spectrum_batch_noalloc",
+    "sr_far_adjoint_from_fields", "sr_spectrum_batch_noalloc", "sr_plan", "sr_autotune_workspace_bytes",
+    "sr_autotune",
 )
 
 SR_NOALLOC_SCRATCH_BYTES = 64 << 20
@@ -95,11 +102,15 @@ class SynchradError(RuntimeError):
     pass
 
 
-def load(path: str = LIB):
-    """Load the library (once) and declare every exported signature."""
+def load(path: str = None):
+    """Load the library (once) and declare every exported signature.
+    ``SYNCHRAD_AMD_LIB`` names another build of the library (for comparing
+    compile-time variants); the default is the in-tree build."""
     global _lib
     if _lib is not None:
         return _lib
+    if path is None:
+        path = os.environ.get("SYNCHRAD_AMD_LIB") or LIB
     if not os.path.exists(path):
         raise SynchradError(
             f"{path} is missing: build it with `python -c 'import __graft_entry__ as g; g.build()'` "
@@ -118,6 +129,18 @@ def load(path: str = LIB):
     L.sr_spectrum_batch.argtypes = [ctypes.POINTER(SrTracks), ctypes.POINTER(SrGrid),
                                     ctypes.POINTER(SrOpts), ctypes.POINTER(_vp), _vp,
                                     ctypes.c_size_t, _vp]
+    L.sr_plan.restype = ctypes.c_int
+    L.sr_plan.argtypes = [ctypes.POINTER(SrTracks), ctypes.POINTER(SrGrid), ctypes.POINTER(SrOpts),
+                          ctypes.POINTER(SrPlanInfo)]
+    L.sr_autotune_workspace_bytes.restype = ctypes.c_int
+    L.sr_autotune_workspace_bytes.argtypes = [ctypes.POINTER(SrTracks), ctypes.POINTER(SrGrid),
+                                              ctypes.POINTER(SrOpts), ctypes.POINTER(SrOpts), ctypes.c_int,
+                                              ctypes.POINTER(ctypes.c_size_t)]
+    L.sr_autotune.restype = ctypes.c_int
+    L.sr_autotune.argtypes = [ctypes.POINTER(SrTracks), ctypes.POINTER(SrGrid), ctypes.POINTER(SrOpts),
+                              ctypes.POINTER(SrOpts), ctypes.c_int, ctypes.POINTER(_vp), _vp, ctypes.c_size_t,
+                              ctypes.c_int, ctypes.c_int, _vp, ctypes.POINTER(ctypes.c_float),
+                              ctypes.POINTER(ctypes.c_int)]
     L.sr_spectrum_batch_noalloc.restype = ctypes.c_int
     L.sr_spectrum_batch_noalloc.argtypes = [ctypes.POINTER(SrTracks), ctypes.POINTER(SrGrid),
                                             ctypes.POINTER(SrOpts), ctypes.POINTER(_vp), _vp]

@@ -46,7 +46,11 @@ namespace {
 
 using namespace sr;
 
-constexpr int kAdjMaxOmega = 512;  // per-omega coefficient table held in LDS
+// The per-omega coefficient tables are held in LDS for at most this many
+// omegas at a time; larger grids are swept in omega tiles (the step's P and R
+// sums are sums over omega, and its gradient words are linear in them, so
+// each tile adds its part)
+constexpr int kAdjOmegaTile = 512;
 constexpr int kSlabWords = 9;      // dL/dx (3), dL/dm (3), dL/da (3) per interval
 
 struct AParams {
@@ -99,8 +103,8 @@ __global__ void __launch_bounds__(kMaxT) k_adjoint(const AParams p) {
   using Acc = typename std::conditional<kSplit, float, double>::type;
   constexpr int NCW = TWO ? 2 : 3;   // coefficient words per omega: re 0..NCW-1, im NCW..2NCW-1
   constexpr int NW = 2 * NCW;
-  __shared__ double s_cw[kAdjMaxOmega];     // omega (F64) or omega/2pi (split)
-  __shared__ float s_cwf[kSplit ? kAdjMaxOmega : 1];   // split: s_cw as the sums' FP32 factor
+  __shared__ double s_cw[kAdjOmegaTile];     // the tile's omega (F64) or omega/2pi (split)
+  __shared__ float s_cwf[kSplit ? kAdjOmegaTile : 1];   // split: s_cw as the sums' FP32 factor
   __shared__ double s_red[kMaxT / kWave];
   extern __shared__ __align__(16) char s_dyn_raw[];
   // H_c F_c of each direction of the batch, re words then im words: split
@@ -135,7 +139,8 @@ __global__ void __launch_bounds__(kMaxT) k_adjoint(const AParams p) {
   const int q = (int)((int64_t)blockIdx.x / p.chunk_n);
   const int64_t ip = p.chunk_base + lp;
   const double dt = p.dt;
-  const int n_om = (int)p.n_omega;
+  // the omega tile [w0, w0 + n_om) whose tables are in LDS
+  int w0 = 0, n_om = 0;
   auto cf_at = [&](int d, int j, int k) -> size_t {
     return kSplit ? ((size_t)d * n_om + j) * NW + k : ((size_t)d * NW + k) * n_om + j;
   };
@@ -168,18 +173,20 @@ __global__ void __launch_bounds__(kMaxT) k_adjoint(const AParams p) {
   // directions are taken in batches whose coefficient tables fit the block's
   // LDS, so that each step's nine gradient words are summed over a batch in
   // registers and written once per batch
-  for (uint32_t d0 = d_lo; d0 < d_hi && n_use > 0; d0 += (uint32_t)p.dir_batch) {
+  for (uint32_t d0 = d_lo; d0 < d_hi && n_use > 0; d0 += (uint32_t)p.dir_batch)
+  for (w0 = 0; w0 < (int)p.n_omega; w0 += kAdjOmegaTile) {
     const int nb = (int)min((uint32_t)p.dir_batch, d_hi - d0);
+    n_om = min(kAdjOmegaTile, (int)p.n_omega - w0);
 
     // ---- coefficient tables from the particle's fields and dL/dS ----
     __syncthreads();   // the previous batch is done with the tables
     for (int idx = tid; idx < nb * n_om; idx += T) {
       const int d = idx / n_om;
-      const int wi = idx - d * n_om;
+      const int wi = idx - d * n_om;      // within the tile
       const uint32_t dir = d0 + (uint32_t)d;
       double nv[3], et[3], ep[3];
       geom(dir, nv, et, ep);
-      const int64_t gidx = (int64_t)dir * p.n_omega + wi;
+      const int64_t gidx = (int64_t)dir * p.n_omega + w0 + wi;
       const int64_t fi = lp * p.n_grid + gidx;
       double Fr[3], Fi[3], G[3];
 #pragma unroll
@@ -205,7 +212,7 @@ __global__ void __launch_bounds__(kMaxT) k_adjoint(const AParams p) {
         }
       }
       if (d == 0) {
-        s_cw[wi] = kSplit ? p.omega[wi] * kInv2Pi : p.omega[wi];
+        s_cw[wi] = kSplit ? p.omega[w0 + wi] * kInv2Pi : p.omega[w0 + wi];
         if (kSplit) s_cwf[wi] = (float)s_cw[wi];
       }
     }
@@ -476,7 +483,7 @@ APlan adjoint_plan(const sr_tracks &t, const sr_grid &g, const sr_opts &o, bool 
   int64_t groups = o.slices > 0 ? o.slices : std::max<int64_t>(1, (4096 + pl.chunk - 1) / pl.chunk);
   pl.groups = (int)std::min<int64_t>(std::min<int64_t>(groups, n_dir), 1 << 16);
   const bool split = o.precision == SR_PREC_SPLIT;
-  const size_t per_dir = (size_t)((o.comp == SR_COMP_TOTAL) ? 4 : 6) * g.n_omega *
+  const size_t per_dir = (size_t)((o.comp == SR_COMP_TOTAL) ? 4 : 6) * std::min<uint32_t>(g.n_omega, kAdjOmegaTile) *
                          (split ? sizeof(float) : sizeof(double));
   const int64_t dirs_per_group = (n_dir + pl.groups - 1) / pl.groups;
   pl.dir_batch = (int)std::max<int64_t>(1, std::min<int64_t>({(int64_t)(kAdjTableBudget / per_dir), 16,
@@ -487,15 +494,13 @@ APlan adjoint_plan(const sr_tracks &t, const sr_grid &g, const sr_opts &o, bool 
 
 int adjoint_check(const sr_tracks *t, const sr_grid *g, const sr_opts *o) {
   if (!t || !g || !o) return fail(SR_ERR_INVALID, "tracks, grid and opts are required");
-  if (g->mode != SR_MODE_FAR) return fail(SR_ERR_UNSUPPORTED, "the adjoint covers the far field only");
+  if (g->mode != SR_MODE_FAR || g->screen_y) return fail(SR_ERR_UNSUPPORTED, "the adjoint covers the far field only");
   if (o->comp != SR_COMP_TOTAL && o->comp != SR_COMP_CARTESIAN)
     return fail(SR_ERR_UNSUPPORTED, "the adjoint covers comp 'total' and 'cartesian' only");
   if (o->precision != SR_PREC_F64 && o->precision != SR_PREC_SPLIT)
     return fail(SR_ERR_INVALID, "precision %d is not F64 (0) or SPLIT (1)", o->precision);
   if (g->n_omega == 0 || g->n_theta == 0 || g->n_phi == 0)
     return fail(SR_ERR_INVALID, "grid sizes must be >= 1");
-  if (g->n_omega > (uint32_t)kAdjMaxOmega)
-    return fail(SR_ERR_UNSUPPORTED, "the adjoint holds at most %d omegas", kAdjMaxOmega);
   if (!g->omega || !g->sin_theta || !g->cos_theta || !g->sin_phi || !g->cos_phi)
     return fail(SR_ERR_INVALID, "grid axis pointer is NULL");
   if (t->n_particles < 0 || t->n_points < 0) return fail(SR_ERR_INVALID, "negative track sizes");

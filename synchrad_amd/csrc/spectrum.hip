@@ -113,6 +113,7 @@ struct KParams {
   int32_t comp;
   uint32_t n_omega, n_theta, n_phi;
   const double *omega, *sin_t, *cos_t, *radius, *sin_p, *cos_p;
+  const double *screen_y;   // near, Cartesian screen: P = (radius[i], screen_y[j], L)
   double l_screen;
   double omega_step;        // rec sweep: omega[i+1] - omega[i]
   const double *form_factor;
@@ -312,7 +313,7 @@ k_spectrum(const KParams p) {
   const int T = blockDim.x;
   const int tid = threadIdx.x;
   const int W = T / kWave;
-  const int wv = tid / kWave;
+  const int wv = uni(tid / kWave);   // wave-uniform: its derived indices stay scalar
   const int lane = tid % kWave;
   // FP64: a tile whose |tau| * om_abs_max reaches kTabMaxPhase / 2 takes the
   // sin/cos table's wide-phase form
@@ -356,35 +357,53 @@ k_spectrum(const KParams p) {
     const uint32_t iTh = dir - iPhi * p.n_theta;
 
     // geometry of the block's point (kernel_farfield.cl:40-42, 384-387;
-    // kernel_nearfield.cl:40-42)
-    double gx, gy, gz;          // far: n ; near: screen point P
-    double thx = 0, thy = 0, thz = 0, phx = 0, phy = 0;
-    if (MODE == SR_MODE_FAR) {
-      const double st = p.sin_t[iTh], ct = p.cos_t[iTh];
-      const double sp = p.sin_p[iPhi], cp = p.cos_p[iPhi];
-      gx = st * cp;
-      gy = st * sp;
-      gz = ct;
-      thx = ct * cp; thy = ct * sp; thz = -st;
-      phx = -sp; phy = cp;
-    } else {
+    // kernel_nearfield.cl:40-42): far n, e_theta, e_phi; near the screen point
+    // P.  Formed where it is used (from the block's uniform axis values, scalar
+    // loads) rather than held in registers through the sweep.
+    // (near: the screen point is held, which measured fewer spilled registers
+    // in the FP64 near-field sweeps)
+    struct Geom {
+      double gx, gy, gz, thx, thy, thz, phx, phy;
+    };
+    double ngx = 0.0, ngy = 0.0, ngz = 0.0;
+    if (MODE == SR_MODE_NEAR) {
       const double r = p.radius[iTh];
-      gx = r * p.cos_p[iPhi];
-      gy = r * p.sin_p[iPhi];
-      gz = p.l_screen;
+      if (p.screen_y) {
+        ngx = r;
+        ngy = p.screen_y[iPhi];
+      } else {
+        ngx = r * p.cos_p[iPhi];
+        ngy = r * p.sin_p[iPhi];
+      }
+      ngz = p.l_screen;
     }
+    auto geom = [&]() {
+      Geom q;
+      if (MODE == SR_MODE_FAR) {
+        const double st = p.sin_t[iTh], ct = p.cos_t[iTh];
+        const double sp = p.sin_p[iPhi], cp = p.cos_p[iPhi];
+        q.gx = st * cp; q.gy = st * sp; q.gz = ct;
+        q.thx = ct * cp; q.thy = ct * sp; q.thz = -st;
+        q.phx = -sp; q.phy = cp;
+      } else {
+        q.gx = ngx; q.gy = ngy; q.gz = ngz;
+        q.thx = q.thy = q.thz = q.phx = q.phy = 0.0;
+      }
+      return q;
+    };
 
     // the block's omegas are dealt to its waves in 64-omega runs; the low runs
     // keep the most steps, so the run -> wave assignment is rotated by the tile
     // index to spread the heavy waves of co-resident blocks over the SIMDs
     const int vw = SR_WAVE_ROTATE ? (int)(((int64_t)wv + tile) % W) : wv;
-    const int tw = vw * kWave + lane;   // this thread's omega slot within a run set
+    // the omega index of the thread's k-th omega: lane of run k*W + vw
+    auto omega_slot = [&](int k) { return (int)(chunk * T * M + (k * W + vw) * kWave + lane); };
     int wi[M];
     bool valid[M];
     double om[M];
 #pragma unroll
     for (int k = 0; k < M; k++) {
-      wi[k] = (int)(chunk * T * M + k * T + tw);
+      wi[k] = omega_slot(k);
       valid[k] = wi[k] < (int)p.n_omega;
       om[k] = valid[k] ? p.omega[wi[k]] : 0.0;
     }
@@ -493,8 +512,9 @@ k_spectrum(const KParams p) {
             re[0] = 0.0; re[1] = F[0]; re[2] = F[1];
             im[0] = 0.0; im[1] = F[2]; im[2] = F[3];
           } else {
-            re[0] = F[0] * thx + F[1] * phx; re[1] = F[0] * thy + F[1] * phy; re[2] = F[0] * thz;
-            im[0] = F[2] * thx + F[3] * phx; im[1] = F[2] * thy + F[3] * phy; im[2] = F[2] * thz;
+            const Geom q = geom();
+            re[0] = F[0] * q.thx + F[1] * q.phx; re[1] = F[0] * q.thy + F[1] * q.phy; re[2] = F[0] * q.thz;
+            im[0] = F[2] * q.thx + F[3] * q.phx; im[1] = F[2] * q.thy + F[3] * q.phy; im[2] = F[2] * q.thz;
           }
         } else {
           // near: the sums are the field, [Re F (3) | Im F (3)]
@@ -547,7 +567,7 @@ k_spectrum(const KParams p) {
         if constexpr (kSplit) {
 #pragma unroll 1
           for (int k = 0; k < M; k++) {
-            const int wk = (int)(chunk * T * M + k * T + tw);
+            const int wk = omega_slot(k);
             if (wk >= (int)p.n_omega) continue;
             double F[NC];
 #pragma unroll
@@ -777,6 +797,9 @@ k_spectrum(const KParams p) {
           const int64_t gs = pt.off + s;
           const double time = (double)(uint32_t)(pt.its + (uint32_t)s) * dt;
           const double xs = p.x[gs], ys = p.y[gs], zs = p.z[gs];
+          const Geom q = geom();
+          const double gx = q.gx, gy = q.gy, gz = q.gz;
+          const double thx = q.thx, thy = q.thy, thz = q.thz, phx = q.phx, phy = q.phy;
           double tau, tau_prev = 0.0;
           if (MODE == SR_MODE_FAR) {
             // kernel_farfield.cl:65-94
@@ -973,7 +996,7 @@ k_spectrum(const KParams p) {
     if (p.acc_lds) {
 #pragma unroll 1
       for (int k = 0; k < M; k++) {
-        const int wk = (int)(chunk * T * M + k * T + tw);
+        const int wk = omega_slot(k);
         if (wk < (int)p.n_omega) dst[0][(int64_t)dir * p.n_omega + wk] += s_acc[k * T];
       }
     }
@@ -1203,10 +1226,14 @@ Plan make_plan(const sr_grid &g, const sr_opts &o, int64_t n_particles, uint32_t
       const int64_t chunks = (n_om + run - 1) / run;
       const double eff = (double)n_om / (double)(chunks * run);
       if (eff < 0.75) continue;
-      // M=2 measured fastest for both sweeps; far-field recurrence blocks of 2
-      // waves (with the tiles launched heaviest first, the finer tiles balance
-      // better: +8 % over 4 waves on the betatron beam), otherwise 4 waves
-      const double mpref = (m == 2) ? 0.02 : 0.0;
+      // far-field recurrence: 4 omegas per thread in 2-wave blocks (one
+      // 512-omega chunk per direction, 161 registers, 3 waves per SIMD:
+      // 5934 G evals/s on the betatron beam against 5267 for 2 omegas x 2
+      // waves and 4871 for 2 x 4); elsewhere M = 2 measured fastest.  Blocks
+      // of 2 waves: with the tiles launched heaviest first, the finer tiles
+      // balance better (+8 % over 4 waves)
+      const bool rec_far = pl.sweep == SWEEP_REC && g.mode == SR_MODE_FAR;
+      const double mpref = rec_far ? (m == 4 ? 0.04 : (m == 2 ? 0.02 : 0.0)) : ((m == 2) ? 0.02 : 0.0);
       const double wpref = (pl.sweep == SWEEP_REC && g.mode == SR_MODE_FAR && w == 2) ? 0.03 : 0.0;
       // launches too small to fill the GPU (one particle: no slices to add
       // waves) prefer the plan with the most waves: 256 CUs hold ~8K waves
@@ -1290,7 +1317,10 @@ int check_grid(const sr_grid *g) {
   if (g->n_omega == 0 || g->n_theta == 0 || g->n_phi == 0)
     return fail(SR_ERR_INVALID, "grid sizes must be >= 1 (got %u, %u, %u)", g->n_omega, g->n_theta,
                 g->n_phi);
-  if (!g->omega || !g->sin_phi || !g->cos_phi) return fail(SR_ERR_INVALID, "grid axis pointer is NULL");
+  if (!g->omega) return fail(SR_ERR_INVALID, "grid axis pointer is NULL");
+  if (g->screen_y && g->mode != SR_MODE_NEAR)
+    return fail(SR_ERR_INVALID, "screen_y (a Cartesian screen) is a near-field grid");
+  if (!g->screen_y && (!g->sin_phi || !g->cos_phi)) return fail(SR_ERR_INVALID, "grid axis pointer is NULL");
   if (g->mode == SR_MODE_FAR && (!g->sin_theta || !g->cos_theta))
     return fail(SR_ERR_INVALID, "far grid needs sin_theta and cos_theta");
   if (g->mode == SR_MODE_NEAR && !g->radius) return fail(SR_ERR_INVALID, "near grid needs radius");
@@ -1337,6 +1367,7 @@ void fill_grid(KParams &kp, const sr_grid &g, const sr_opts &o, int n_out, doubl
   kp.sin_p = g.sin_phi;
   kp.cos_p = g.cos_phi;
   kp.l_screen = g.l_screen;
+  kp.screen_y = g.screen_y;
   kp.omega_step = g.omega_step;
   kp.form_factor = o.form_factor;
   kp.n_grid = (int64_t)g.n_omega * g.n_theta * g.n_phi;
@@ -1630,6 +1661,99 @@ int sr_spectrum_batch(const sr_tracks *t, const sr_grid *g, const sr_opts *o, do
              order_bytes(pl, *g, *o, t->n_points));
 }
 
+int sr_plan(const sr_tracks *t, const sr_grid *g, const sr_opts *o, sr_plan_info *info) {
+  int rc;
+  if (!info) return fail(SR_ERR_INVALID, "info is NULL");
+  memset(info, 0, sizeof(*info));
+  if ((rc = check_tracks(t)) || (rc = check_grid(g)) || (rc = check_opts(g, o))) return rc;
+  const Plan pl = make_plan(*g, *o, std::max<int64_t>(t->n_particles, 1), t->n_snaps, t->n_points);
+  info->sweep = pl.sweep == SWEEP_REC ? SR_SWEEP_RECURRENCE : SR_SWEEP_DIRECT;
+  info->omega_per_lane = pl.M;
+  info->waves_per_block = pl.waves;
+  info->slices = pl.slices;
+  info->tiles = pl.blocks;
+  info->ordered = order_bytes(pl, *g, *o, t->n_points) > 0 ? 1 : 0;
+  return SR_OK;
+}
+
+namespace {
+// the candidate's plan fields over the base request's output and precision
+sr_opts candidate_opts(const sr_opts &base, const sr_opts &cand) {
+  sr_opts o = cand;
+  o.comp = base.comp;
+  o.precision = base.precision;
+  o.form_factor = base.form_factor;
+  return o;
+}
+}  // namespace
+
+int sr_autotune_workspace_bytes(const sr_tracks *t, const sr_grid *g, const sr_opts *base,
+                                const sr_opts *candidates, int n_candidates, size_t *bytes) {
+  if (!bytes) return fail(SR_ERR_INVALID, "bytes is NULL");
+  *bytes = 0;
+  if (!base || !candidates || n_candidates <= 0) return fail(SR_ERR_INVALID, "no candidates");
+  for (int i = 0; i < n_candidates; i++) {
+    const sr_opts o = candidate_opts(*base, candidates[i]);
+    size_t b = 0;
+    if (sr_workspace_bytes(t, g, &o, &b) == SR_OK) *bytes = std::max(*bytes, b);
+  }
+  return SR_OK;
+}
+
+int sr_autotune(const sr_tracks *t, const sr_grid *g, const sr_opts *base, const sr_opts *candidates,
+                int n_candidates, double *const *outs, void *workspace, size_t workspace_bytes, int warmup,
+                int reps, void *stream, float *ms, int *best) {
+  int rc;
+  if (!base || !candidates || n_candidates <= 0 || !ms || !best)
+    return fail(SR_ERR_INVALID, "sr_autotune needs a base request, candidates, ms[] and best");
+  if ((rc = check_tracks(t)) || (rc = check_grid(g)) || (rc = check_opts(g, base))) return rc;
+  if (reps < 1 || warmup < 0) return fail(SR_ERR_INVALID, "reps must be >= 1 and warmup >= 0");
+  hipStream_t st = (hipStream_t)stream;
+  hipEvent_t e0, e1;
+  hipError_t e = hipEventCreate(&e0);
+  if (e != hipSuccess) return fail(SR_ERR_HIP, "hipEventCreate: %s", hipGetErrorString(e));
+  e = hipEventCreate(&e1);
+  if (e != hipSuccess) {
+    hipEventDestroy(e0);
+    return fail(SR_ERR_HIP, "hipEventCreate: %s", hipGetErrorString(e));
+  }
+  *best = -1;
+  float best_ms = INFINITY;
+  rc = SR_OK;
+  for (int i = 0; i < n_candidates && rc == SR_OK; i++) {
+    ms[i] = INFINITY;
+    const sr_opts o = candidate_opts(*base, candidates[i]);
+    size_t need = 0;
+    if (check_opts(g, &o) != SR_OK || sr_workspace_bytes(t, g, &o, &need) != SR_OK || need > workspace_bytes)
+      continue;   // a plan this request cannot run (or cannot fit) is not a candidate
+    // untimed runs (code objects, caches), then reps timed ones
+    for (int r = 0; r < warmup && rc == SR_OK; r++)
+      rc = sr_spectrum_batch(t, g, &o, outs, workspace, workspace_bytes, stream);
+    if (rc != SR_OK) break;
+    hipEventRecord(e0, st);
+    for (int r = 0; r < reps && rc == SR_OK; r++)
+      rc = sr_spectrum_batch(t, g, &o, outs, workspace, workspace_bytes, stream);
+    hipEventRecord(e1, st);
+    e = hipEventSynchronize(e1);
+    if (e != hipSuccess) {
+      rc = fail(SR_ERR_HIP, "timing candidate %d: %s", i, hipGetErrorString(e));
+      break;
+    }
+    if (rc != SR_OK) break;
+    float elapsed = 0.f;
+    hipEventElapsedTime(&elapsed, e0, e1);
+    ms[i] = elapsed / (float)reps;
+    if (ms[i] < best_ms) {
+      best_ms = ms[i];
+      *best = i;
+    }
+  }
+  hipEventDestroy(e0);
+  hipEventDestroy(e1);
+  if (rc == SR_OK && *best < 0) rc = fail(SR_ERR_INVALID, "no candidate plan can run this request");
+  return rc;
+}
+
 int sr_spectrum_batch_noalloc(const sr_tracks *t, const sr_grid *g, const sr_opts *o, double *const *outs,
                               void *stream) {
   int rc;
@@ -1674,7 +1798,6 @@ int sr_spectrum_batch_noalloc(const sr_tracks *t, const sr_grid *g, const sr_opt
   return run(kp, pl, *g, *o, scr, kNoAllocScratchBytes, (hipStream_t)stream, false,
              order_bytes(pl, *g, *o, t->n_points));
 }
-
 
 int sr_far_total(double *spectrum, const double *x, const double *y, const double *z,
                  const double *ux, const double *uy, const double *uz, double wp, uint32_t itStart,

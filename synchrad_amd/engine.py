@@ -150,25 +150,59 @@ def pack_tracks(tracks: Sequence[Sequence], *, dt: float, it_range=None, nSnaps:
 # ---------------------------------------------------------------------------
 # device objects
 # ---------------------------------------------------------------------------
+_PACK_ALIGN = 256
+
+
+def _packed_layout(hb: HostBatch):
+    """Byte offsets of a batch's arrays in one contiguous buffer: offsets
+    (int64), the six coordinate and momentum arrays and the weights (float64),
+    it_start, it_end and it_snaps (uint32), each 256-byte aligned."""
+    parts = [("offsets", hb.offsets.size * 8)] + [(c, hb.n_points * 8) for c in ("x", "y", "z", "ux", "uy", "uz")] + \
+        [("weight", hb.n_particles * 8), ("it_start", hb.n_particles * 4), ("it_end", hb.n_particles * 4),
+         ("it_snaps", hb.it_snaps.size * 4)]
+    layout, at = {}, 0
+    for name, nbytes in parts:
+        layout[name] = (at, nbytes)
+        at += (nbytes + _PACK_ALIGN - 1) // _PACK_ALIGN * _PACK_ALIGN
+    return layout, max(at, _PACK_ALIGN)
+
+
+def _pack_host(hb: HostBatch, layout, total) -> np.ndarray:
+    buf = np.empty(total, dtype=np.uint8)
+    for name, (at, nbytes) in layout.items():
+        if name in ("it_start", "it_end", "it_snaps"):
+            arr = np.ascontiguousarray(getattr(hb, name), dtype=np.uint32)
+        elif name == "offsets":
+            arr = np.ascontiguousarray(hb.offsets, dtype=np.int64)
+        else:
+            arr = np.ascontiguousarray(getattr(hb, name), dtype=np.float64)
+        buf[at:at + nbytes] = arr.view(np.uint8)
+    return buf
+
+
 class DeviceBatch:
-    """A HostBatch uploaded to the HIP device (one copy per array).
+    """A HostBatch on the HIP device: every array in one device buffer, filled
+    by one host-to-device copy.
 
     ``tensors`` may supply the coordinate, momentum and weight arrays as
     existing device tensors (float64, contiguous; x, y, z, ux, uy, uz, weight),
-    which are then used in place -- the autograd path does this."""
+    which are then used in place -- the autograd path does this.  ``refill``
+    loads another batch of the same sizes into the same device memory (so a
+    captured launch sequence sees the new tracks)."""
 
     def __init__(self, hb: HostBatch, device=None, tensors: Optional[Sequence[torch.Tensor]] = None):
         dev = _dev(device)
         self.host = hb
         self.device = dev
-
-        def f64(a):
-            return torch.from_numpy(np.ascontiguousarray(a, dtype=np.float64)).to(dev)
-
-        def u32(a):
-            return torch.from_numpy(np.ascontiguousarray(a, dtype=np.uint32).view(np.int32)).to(dev)
-
-        self.offsets = torch.from_numpy(hb.offsets).to(dev)
+        self._layout, total = _packed_layout(hb)
+        self._buf = torch.from_numpy(_pack_host(hb, self._layout, total)).to(dev)
+        views = {}
+        for name, (at, nbytes) in self._layout.items():
+            raw = self._buf[at:at + nbytes]
+            views[name] = raw.view(torch.int64 if name == "offsets" else
+                                   torch.int32 if name.startswith("it_") else torch.float64)
+        self.offsets = views["offsets"]
+        self.it_start, self.it_end, self.it_snaps = views["it_start"], views["it_end"], views["it_snaps"]
         if tensors is not None:
             ts = list(tensors)
             for t, n in zip(ts, [hb.n_points] * 6 + [hb.n_particles]):
@@ -176,12 +210,27 @@ class DeviceBatch:
                     raise ValueError("track tensors must be contiguous float64 on the batch device")
             self.x, self.y, self.z, self.ux, self.uy, self.uz, self.weight = ts
         else:
-            self.x, self.y, self.z = f64(hb.x), f64(hb.y), f64(hb.z)
-            self.ux, self.uy, self.uz = f64(hb.ux), f64(hb.uy), f64(hb.uz)
-            self.weight = f64(hb.weight)
-        self.it_start = u32(hb.it_start)
-        self.it_end = u32(hb.it_end)
-        self.it_snaps = u32(hb.it_snaps)
+            self.x, self.y, self.z = views["x"], views["y"], views["z"]
+            self.ux, self.uy, self.uz = views["ux"], views["uy"], views["uz"]
+            self.weight = views["weight"]
+        self._offsets_host = np.ascontiguousarray(hb.offsets, dtype=np.int64)
+
+    def shape_key(self):
+        """What a refill must keep: sample and particle counts, snapshot list
+        form and the time step (the library plans and captures on these)."""
+        hb = self.host
+        return (hb.n_particles, hb.n_points, hb.n_snaps, hb.snaps_stride, int(hb.it_snaps.size), hb.dt)
+
+    def refill(self, hb: HostBatch) -> None:
+        """Load ``hb`` (same shape_key) into this batch's device memory."""
+        old = self.host
+        self.host = hb
+        if self.shape_key() != (old.n_particles, old.n_points, old.n_snaps, old.snaps_stride,
+                                int(old.it_snaps.size), old.dt):
+            self.host = old
+            raise ValueError("refill needs a batch of the same sizes, snapshot form and time step")
+        layout, total = _packed_layout(hb)
+        self._buf.copy_(torch.from_numpy(_pack_host(hb, layout, total)))
         self._offsets_host = np.ascontiguousarray(hb.offsets, dtype=np.int64)
 
     @property
@@ -210,14 +259,22 @@ class DeviceBatch:
 class DeviceGrid:
     """Spectral axes on the device, formed as SynchRad._init_data forms them
     (synchrad/calc.py:494-512): omega times 2*pi, and numpy sin/cos of the
-    double-precision angle axes."""
+    double-precision angle axes.
+
+    ``screen='cartesian'`` (near field only; the reference's screen is polar)
+    takes ``ax1`` as the screen's x positions and ``phi`` as its y positions:
+    the observer of (x_i, y_j) is (x_i, y_j, L_screen), and the spectra are
+    laid out (nSnaps, n_y, n_x, nOmega)."""
 
     def __init__(self, mode: str, omega: np.ndarray, ax1: np.ndarray, phi: np.ndarray,
-                 L_screen: Optional[float] = None, device=None):
+                 L_screen: Optional[float] = None, device=None, screen: str = "polar"):
         dev = _dev(device)
         if mode not in _abi.MODES:
             raise ValueError(f"mode must be 'far' or 'near', got {mode!r}")
+        if screen not in ("polar", "cartesian") or (screen == "cartesian" and mode != "near"):
+            raise ValueError("screen must be 'polar', or 'cartesian' in the near field")
         self.mode = mode
+        self.screen = screen
         self.device = dev
         omega = np.asarray(omega, dtype=np.float64)
         ax1 = np.asarray(ax1, dtype=np.float64)
@@ -231,8 +288,13 @@ class DeviceGrid:
         w = np.double(2 * np.pi) * omega
         self.omega = f64(w)
         self.omega_step = uniform_step(w)
-        self.sin_phi = f64(np.sin(phi))
-        self.cos_phi = f64(np.cos(phi))
+        self.screen_y = None
+        if screen == "cartesian":
+            self.sin_phi = self.cos_phi = None
+            self.screen_y = f64(phi)
+        else:
+            self.sin_phi = f64(np.sin(phi))
+            self.cos_phi = f64(np.cos(phi))
         self.sin_theta = self.cos_theta = self.radius = None
         if mode == "far":
             self.sin_theta = f64(np.sin(ax1))
@@ -253,8 +315,11 @@ class DeviceGrid:
         g.mode = _abi.MODES[self.mode]
         g.n_omega, g.n_theta, g.n_phi = self.n_omega, self.n1, self.n_phi
         g.omega = self.omega.data_ptr()
-        g.sin_phi = self.sin_phi.data_ptr()
-        g.cos_phi = self.cos_phi.data_ptr()
+        if self.screen_y is not None:
+            g.screen_y = self.screen_y.data_ptr()
+        else:
+            g.sin_phi = self.sin_phi.data_ptr()
+            g.cos_phi = self.cos_phi.data_ptr()
         if self.mode == "far":
             g.sin_theta = self.sin_theta.data_ptr()
             g.cos_theta = self.cos_theta.data_ptr()
@@ -326,8 +391,7 @@ def spectrum(batch: DeviceBatch, grid: DeviceGrid, comp: str = "total", precisio
     Stream-ordered on the current (or given) torch stream; no synchronisation.
 
     ``tuning``: None (the library's own plan), a :class:`Tuning`, or "auto"
-    (the autotuner's plan for this batch and grid, timed once and cached;
-    the library's plan for launches the autotuner's samples cannot rank).
+    (the autotuner's plan for this batch and grid, timed once and cached).
     ``workspace``: a preallocated uint8 device tensor to use instead of
     allocating one (needed under graph capture)."""
     L = _abi.load()
@@ -336,9 +400,7 @@ def spectrum(batch: DeviceBatch, grid: DeviceGrid, comp: str = "total", precisio
             raise ValueError("tuning must be None, a Tuning or 'auto'")
         from . import autotune
         prec_name = precision if isinstance(precision, str) else ("double" if precision == 0 else "float")
-        # launches no sample can rank (large far-field ones) keep the library's plan
-        tuning = autotune.tune(batch, grid, comp=comp, precision=prec_name)[0] \
-            if autotune.representative(batch, grid) else None
+        tuning = autotune.tune(batch, grid, comp=comp, precision=prec_name)[0]
     if comp not in _abi.COMPS:
         raise ValueError(f"unknown comp {comp!r}")
     prec = PRECISIONS[precision] if isinstance(precision, str) else int(precision)
@@ -426,12 +488,11 @@ class SpectrumGraph:
     launch."""
 
     def __init__(self, batch: DeviceBatch, grid: DeviceGrid, comp: str = "total", precision="float",
-                 tuning: Optional[Tuning] = None, sigma_particle: float = 0.0):
+                 tuning: Optional[Tuning] = None, sigma_particle: float = 0.0, warm: bool = True):
         self.batch, self.grid, self.comp, self.precision = batch, grid, comp, precision
         if isinstance(tuning, str):
             from . import autotune
-            tuning = autotune.tune(batch, grid, comp=comp, precision=precision)[0] \
-                if autotune.representative(batch, grid) else None
+            tuning = autotune.tune(batch, grid, comp=comp, precision=precision)[0]
         self.tuning = tuning or Tuning()
         L = _abi.load()
         n_out = L.sr_num_outputs(_abi.MODES[grid.mode], _abi.COMPS[comp])
@@ -441,12 +502,14 @@ class SpectrumGraph:
         self.workspace = torch.empty(max(nb, 1), dtype=torch.uint8, device=grid.device)
         self.ff = grid.form_factor(sigma_particle) if (comp == "cartesian_complex" and grid.mode == "far") else None
         self.sigma = sigma_particle
-        # warm up on a side stream (loads the code objects), then capture
-        s = torch.cuda.Stream(device=grid.device)
-        s.wait_stream(torch.cuda.current_stream(grid.device))
-        with torch.cuda.stream(s):
-            self._body()
-        torch.cuda.current_stream(grid.device).wait_stream(s)
+        # warm up on a side stream (loads the code objects), then capture;
+        # warm=False when this process already ran the same launch
+        if warm:
+            s = torch.cuda.Stream(device=grid.device)
+            s.wait_stream(torch.cuda.current_stream(grid.device))
+            with torch.cuda.stream(s):
+                self._body()
+            torch.cuda.current_stream(grid.device).wait_stream(s)
         torch.cuda.synchronize(grid.device)
         self.graph = torch.cuda.CUDAGraph()
         with torch.cuda.graph(self.graph):

@@ -475,3 +475,22 @@ def test_autograd_with_and_without_kept_fields(monkeypatch):
     assert float(torch.linalg.norm(S_keep - S_re) / torch.linalg.norm(S_re)) < 1e-14
     for a, b in zip(g_keep, g_re):
         assert float(torch.linalg.norm(a - b) / torch.linalg.norm(b)) < 1e-12
+
+
+@pytest.mark.parametrize("comp", ["total", "cartesian"])
+@pytest.mark.parametrize("precision,tol", [("double", 1e-10), ("float", 1e-4)])
+def test_adjoint_beyond_one_omega_tile(comp, precision, tol):
+    """A 1100-omega grid: the coefficient tables are held 512 omegas at a time
+    (three tiles, the last partial), and the gradients still equal the
+    oracle adjoint's."""
+    wl = workloads.config5(n_particles=2)
+    tr = [[a[:240] for a in t[:6]] + [t[6] * (1.0 + 0.4 * i), 0] for i, t in enumerate(wl.track_list())]
+    axes = oracle.make_axes([(1.0 * KEV, 30.0 * KEV), (0.0, 0.02), (0.0, 2 * np.pi), (1100, 3, 2)], "far")
+    dt = wl.batch.dt
+    keys = oracle.COMP_KEYS[comp]
+    S0 = oracle.spectrum("far", comp, tr, axes, dt)
+    rng = np.random.default_rng(17)
+    G = {k: rng.standard_normal(S0[k].shape) * S0[k].mean() for k in keys}
+    ref, ref_w = oracle.far_adjoint(comp, tr, axes, dt, G)
+    per, gw = gpu_grads(tr, axes, dt, G, comp, precision)
+    compare(per, gw, ref, ref_w, tol, f"1100 omega {comp} {precision}")
