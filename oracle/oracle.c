@@ -198,33 +198,13 @@ int oracle_far(int comp, double *const *spectra,
  * kernel declares a FormFactor argument but never reads it
  * (kernel_nearfield.cl:216-326); it is ignored here as well.
  */
-int oracle_near(int comp, double *const *spectra,
-                const double *x, const double *y, const double *z,
-                const double *ux, const double *uy, const double *uz,
-                double wp, uint32_t itStart, uint32_t itEnd, uint32_t nSteps,
-                const double *omega, const double *radius,
-                const double *sinPhi, const double *cosPhi, double distanceToScreen,
-                uint32_t nOmega, uint32_t nRadius, uint32_t nPhi,
-                double dt, uint32_t nSnaps, const uint32_t *itSnaps, int nthreads) {
-  if (!(comp == OR_TOTAL || comp == OR_CARTESIAN || comp == OR_CARTESIAN_COMPLEX)) return -1;
-  if (itEnd == 0 || nSteps == 0) return -2;
-  const int64_t nTotal = (int64_t)nRadius * nPhi * nOmega;
-#ifdef _OPENMP
-  if (nthreads > 0) omp_set_num_threads(nthreads);
-#else
-  (void)nthreads;
-#endif
-
-#pragma omp parallel for schedule(static)
-  for (int64_t gti = 0; gti < nTotal; gti++) {
-    /* kernel_nearfield.cl:34-42 */
-    const uint32_t iPhi = (uint32_t)(gti / ((int64_t)nOmega * nRadius));
-    const uint32_t iRadius = (uint32_t)((gti - (int64_t)iPhi * nOmega * nRadius) / nOmega);
-    const uint32_t iOmega = (uint32_t)(gti - (int64_t)iPhi * nOmega * nRadius - (int64_t)iRadius * nOmega);
-    const double omegaLocal = omega[iOmega];
-    const v3 scr = mk(radius[iRadius] * cosPhi[iPhi], radius[iRadius] * sinPhi[iPhi],
-                      distanceToScreen);
-
+/* one screen point scr and frequency omegaLocal of the near field: the body of
+ * kernel_nearfield.cl:44-101 for spectral index gti */
+static void near_point(int comp, double *const *spectra, int64_t gti, int64_t nTotal, v3 scr, double omegaLocal,
+                       const double *x, const double *y, const double *z,
+                       const double *ux, const double *uy, const double *uz,
+                       double wp, uint32_t itStart, uint32_t itEnd, uint32_t nSteps,
+                       double dt, uint32_t nSnaps, const uint32_t *itSnaps) {
     const double wpdt2 = wp * dt * dt;
     const double wpdt = sqrt(wp) * dt;
     double phasePrev = 0.0;
@@ -262,6 +242,69 @@ int oracle_near(int comp, double *const *spectra,
         iSnap += 1;
       }
     }
+}
+
+int oracle_near(int comp, double *const *spectra,
+                const double *x, const double *y, const double *z,
+                const double *ux, const double *uy, const double *uz,
+                double wp, uint32_t itStart, uint32_t itEnd, uint32_t nSteps,
+                const double *omega, const double *radius,
+                const double *sinPhi, const double *cosPhi, double distanceToScreen,
+                uint32_t nOmega, uint32_t nRadius, uint32_t nPhi,
+                double dt, uint32_t nSnaps, const uint32_t *itSnaps, int nthreads) {
+  if (!(comp == OR_TOTAL || comp == OR_CARTESIAN || comp == OR_CARTESIAN_COMPLEX)) return -1;
+  if (itEnd == 0 || nSteps == 0) return -2;
+  const int64_t nTotal = (int64_t)nRadius * nPhi * nOmega;
+#ifdef _OPENMP
+  if (nthreads > 0) omp_set_num_threads(nthreads);
+#else
+  (void)nthreads;
+#endif
+
+#pragma omp parallel for schedule(static)
+  for (int64_t gti = 0; gti < nTotal; gti++) {
+    /* kernel_nearfield.cl:34-42 */
+    const uint32_t iPhi = (uint32_t)(gti / ((int64_t)nOmega * nRadius));
+    const uint32_t iRadius = (uint32_t)((gti - (int64_t)iPhi * nOmega * nRadius) / nOmega);
+    const uint32_t iOmega = (uint32_t)(gti - (int64_t)iPhi * nOmega * nRadius - (int64_t)iRadius * nOmega);
+    const v3 scr = mk(radius[iRadius] * cosPhi[iPhi], radius[iRadius] * sinPhi[iPhi],
+                      distanceToScreen);
+    near_point(comp, spectra, gti, nTotal, scr, omega[iOmega], x, y, z, ux, uy, uz, wp, itStart, itEnd, nSteps,
+               dt, nSnaps, itSnaps);
+  }
+  return 0;
+}
+
+/*
+ * Near field on a Cartesian screen (no counterpart in the reference, whose
+ * screen is polar): the observer of index (iX, iY) is (screenX[iX],
+ * screenY[iY], L), laid out like the polar screen with x in place of the
+ * radius and y in place of phi; everything else is near_point above.
+ */
+int oracle_near_xy(int comp, double *const *spectra,
+                   const double *x, const double *y, const double *z,
+                   const double *ux, const double *uy, const double *uz,
+                   double wp, uint32_t itStart, uint32_t itEnd, uint32_t nSteps,
+                   const double *omega, const double *screenX, const double *screenY, double distanceToScreen,
+                   uint32_t nOmega, uint32_t nX, uint32_t nY,
+                   double dt, uint32_t nSnaps, const uint32_t *itSnaps, int nthreads) {
+  if (!(comp == OR_TOTAL || comp == OR_CARTESIAN || comp == OR_CARTESIAN_COMPLEX)) return -1;
+  if (itEnd == 0 || nSteps == 0) return -2;
+  const int64_t nTotal = (int64_t)nX * nY * nOmega;
+#ifdef _OPENMP
+  if (nthreads > 0) omp_set_num_threads(nthreads);
+#else
+  (void)nthreads;
+#endif
+
+#pragma omp parallel for schedule(static)
+  for (int64_t gti = 0; gti < nTotal; gti++) {
+    const uint32_t iY = (uint32_t)(gti / ((int64_t)nOmega * nX));
+    const uint32_t iX = (uint32_t)((gti - (int64_t)iY * nOmega * nX) / nOmega);
+    const uint32_t iOmega = (uint32_t)(gti - (int64_t)iY * nOmega * nX - (int64_t)iX * nOmega);
+    const v3 scr = mk(screenX[iX], screenY[iY], distanceToScreen);
+    near_point(comp, spectra, gti, nTotal, scr, omega[iOmega], x, y, z, ux, uy, uz, wp, itStart, itEnd, nSteps,
+               dt, nSnaps, itSnaps);
   }
   return 0;
 }

@@ -67,6 +67,12 @@ def lib(path: Optional[str] = None):
             + [ctypes.c_double, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32]
             + [_dp] * 5 + [ctypes.c_uint32] * 3
             + [ctypes.c_double, ctypes.c_uint32, _up, _dp, ctypes.c_int])
+        L.oracle_near_xy.restype = ctypes.c_int
+        L.oracle_near_xy.argtypes = (
+            [ctypes.c_int, ctypes.POINTER(_dp)] + [_dp] * 6
+            + [ctypes.c_double, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32]
+            + [_dp] * 3 + [ctypes.c_double] + [ctypes.c_uint32] * 3
+            + [ctypes.c_double, ctypes.c_uint32, _up, ctypes.c_int])
         L.oracle_near.restype = ctypes.c_int
         L.oracle_near.argtypes = (
             [ctypes.c_int, ctypes.POINTER(_dp)] + [_dp] * 6
@@ -123,18 +129,32 @@ def spectrum(mode: str, comp: str, tracks: List[list], axes: Dict[str, np.ndarra
     """Incoherent/coherent spectra of ``tracks`` on ``axes`` (FP64).
 
     ``tracks`` items are ``[x, y, z, ux, uy, uz, w, it_start]`` or the 7-item
-    form without ``it_start`` (synchrad/calc.py:580-584).  Returns
+    form without ``it_start`` (synchrad/calc.py:580-584).  Near-field
+    ``axes`` with "x" and "y" (and no "radius") describe a Cartesian screen,
+    an extension the reference does not have (``oracle_near_xy``).  Returns
     ``{key: array(nSnaps, nOmega, nTheta|nR, nPhi)}`` as
     ``SynchRad.Data['radiation']`` holds it (synchrad/calc.py:573-577).
     """
     L = library if library is not None else lib()
     omega = axes["omega"].astype(np.double)
-    phi = axes["phi"].astype(np.double)
     omega_dev = np.ascontiguousarray(np.double(2 * np.pi) * omega)
-    sinP = np.ascontiguousarray(np.sin(phi))
-    cosP = np.ascontiguousarray(np.cos(phi))
+    cartesian = mode == "near" and "x" in axes and "y" in axes
+    if cartesian:
+        # Cartesian near-field screen (no counterpart in the reference): x in
+        # place of the radius, y in place of phi
+        scr_x = np.ascontiguousarray(axes["x"], dtype=np.double)
+        scr_y = np.ascontiguousarray(axes["y"], dtype=np.double)
+        phi = scr_y
+    else:
+        phi = axes["phi"].astype(np.double)
+        sinP = np.ascontiguousarray(np.sin(phi))
+        cosP = np.ascontiguousarray(np.cos(phi))
     nO, nP = omega.size, phi.size
-    if mode == "far":
+    if cartesian:
+        if L_screen is None:
+            raise ValueError("near field needs L_screen")
+        n1 = scr_x.size
+    elif mode == "far":
         theta = axes["theta"].astype(np.double)
         sinT = np.ascontiguousarray(np.sin(theta))
         cosT = np.ascontiguousarray(np.cos(theta))
@@ -183,6 +203,12 @@ def spectrum(mode: str, comp: str, tracks: List[list], axes: Dict[str, np.ndarra
                               _p(omega_dev), _p(sinT), _p(cosT), _p(sinP), _p(cosP),
                               nO, n1, nP, float(timeStep), nSnaps,
                               snaps.ctypes.data_as(_up), _p(ff), int(threads))
+        elif cartesian:
+            rc = L.oracle_near_xy(code, out_ptrs, *[_p(a) for a in arrs],
+                                  float(wp), int(np.uint32(it_start)), int(itEnd), int(n),
+                                  _p(omega_dev), _p(scr_x), _p(scr_y), float(L_screen),
+                                  nO, n1, nP, float(timeStep), nSnaps,
+                                  snaps.ctypes.data_as(_up), int(threads))
         else:
             rc = L.oracle_near(code, out_ptrs, *[_p(a) for a in arrs],
                                float(wp), int(np.uint32(it_start)), int(itEnd), int(n),

@@ -49,10 +49,27 @@ class Workload:
         else:
             omega = np.r_[omin:omax:No * 1j]
         ax1 = np.r_[g[1][0]:g[1][1]:N1 * 1j]
+        if self.screen == "cartesian":
+            # a Cartesian near-field screen: x and y both inclusive
+            y = np.r_[g[2][0]:g[2][1]:Nph * 1j]
+            return {"omega": omega.astype(np.double), "x": ax1.astype(np.double), "y": y.astype(np.double)}
         phi = g[2][0] + (g[2][1] - g[2][0]) / Nph * np.arange(Nph)
         key = "theta" if self.mode == "far" else "radius"
         return {"omega": omega.astype(np.double), key: ax1.astype(np.double),
                 "phi": phi.astype(np.double)}
+
+    @property
+    def screen(self) -> str:
+        return self.extra.get("screen", "polar")
+
+    def device_grid(self, device=None):
+        """The workload's grid as an engine.DeviceGrid."""
+        ax = self.axes
+        if self.screen == "cartesian":
+            return engine.DeviceGrid("near", ax["omega"], ax["x"], ax["y"], L_screen=self.L_screen, device=device,
+                                     screen="cartesian")
+        ax1 = ax["theta"] if self.mode == "far" else ax["radius"]
+        return engine.DeviceGrid(self.mode, ax["omega"], ax1, ax["phi"], L_screen=self.L_screen, device=device)
 
     @property
     def n_grid(self) -> int:
@@ -234,11 +251,13 @@ def config3(scale: float = 1.0, n_particles: Optional[int] = None) -> Workload:
     return Workload("uneven-csr", hb, ci, "cartesian")
 
 
-def config4(scale: float = 1.0, n_particles: Optional[int] = None) -> Workload:
+def config4(scale: float = 1.0, n_particles: Optional[int] = None, screen: str = "polar") -> Workload:
     """Near field: 500 e-, planar undulator gamma = 200(1 + N(0, 0.5%)), K=2,
     lambda_u = 2 cm, 100 periods x 40 steps; offsets N(0, 20 um), angles N(0, 10 urad);
     screen at L = 7 m (5 m past the exit); 128 omega on [0.8, 1.2] w1 x 64 R on
-    [0, 7.5 cm] x 64 phi; FP64 distance, FP32 amplitude."""
+    [0, 7.5 cm] x 64 phi (the reference's polar screen), or with
+    screen='cartesian' 64 x 64 points over +-7.5 cm (+-3/gamma at 5 m); FP64
+    distance, FP32 amplitude."""
     rng = np.random.default_rng(1004)
     Np_full = 500
     g0, K, lu = 200.0, 2.0, 0.02
@@ -259,6 +278,11 @@ def config4(scale: float = 1.0, n_particles: Optional[int] = None) -> Workload:
     hb = engine.pack_csr(_csr(np.full(Np, n)), *cols, np.ones(Np), dt=cdt)
     w1 = undulator_omega1(g0, K, lu)
     no, nr, nph = (max(8, int(128 * scale)), max(4, int(64 * scale)), max(4, int(64 * scale)))
+    if screen == "cartesian":
+        ci = {"grid": [(0.8 * w1, 1.2 * w1), (-0.075, 0.075), (-0.075, 0.075), (no, nr, nph)],
+              "dtype": "float", "mode": "near"}
+        return Workload("near-undulator-xy", hb, ci, "total", mode="near", L_screen=7.0,
+                        extra={"omega1": w1, "screen": "cartesian"})
     ci = {"grid": [(0.8 * w1, 1.2 * w1), (0.0, 0.075), (0.0, 2 * np.pi), (no, nr, nph)],
           "dtype": "float", "mode": "near"}
     return Workload("near-undulator", hb, ci, "total", mode="near", L_screen=7.0,

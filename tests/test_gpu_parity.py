@@ -338,7 +338,7 @@ def test_autotuned_plan_matches_oracle():
     ax = wl.axes
     grid = engine.DeviceGrid("far", ax["omega"], ax["theta"], ax["phi"])
     plan, rows = autotune.tune(batch, grid, comp="total", precision="float", target_evals=1e8)
-    assert len(rows) >= 6 and all(r["ms"] > 0 for r in rows)
+    assert len(rows) == len(autotune.candidates("far", "float", True, 128)) and all(r["ms"] > 0 for r in rows)
     out = engine.spectrum(batch, grid, comp="total", precision="float", tuning=plan)[0]
     got = out.cpu().numpy().swapaxes(-1, -3)
     assert rel_l2(got, ref["total"]) < SPLIT_TOL
@@ -469,3 +469,19 @@ def test_zero_phase_step_is_kept(sweep):
     r = rel_l2(got, ref)
     print(f"zero-phase steps, {sweep}: on-axis rel-L2 {r_axis:.3e}, all {r:.3e}", flush=True)
     assert r_axis <= SPLIT_TOL and r <= SPLIT_TOL
+
+
+@pytest.mark.parametrize("comp", ["total", "cartesian"])
+@pytest.mark.parametrize("precision,sweep", [("double", "auto"), ("float", "direct"), ("float", "recurrence")])
+def test_near_field_cartesian_screen(comp, precision, sweep):
+    """The 64 x 64 Cartesian screen of the near-field workload (scaled): HIP
+    against the oracle's Cartesian-screen restatement."""
+    wl = workloads.config4(scale=0.25, n_particles=3, screen="cartesian")
+    tr = wl.track_list()
+    ref = oracle.spectrum("near", comp, tr, wl.axes, wl.batch.dt, L_screen=wl.L_screen)
+    batch = engine.DeviceBatch(engine.pack_tracks(tr, dt=wl.batch.dt))
+    outs = engine.spectrum(batch, wl.device_grid(), comp=comp, precision=precision,
+                           tuning=engine.Tuning(sweep=sweep))
+    torch.cuda.synchronize()
+    got = {k: o.cpu().numpy().swapaxes(-1, -3) for k, o in zip(engine.COMP_KEYS[comp], outs)}
+    check(got, ref, precision, "near xy " + comp)

@@ -127,16 +127,10 @@ def test_weight_normalisation_mutates_caller_lists(monkeypatch):
     calc.device = "cpu-test"
     seen = {}
 
-    def fake_spectrum(batch, grid, **kw):
-        seen["w"] = batch.host.weight.copy()
-        return kw["outs"]
+    def fake_integrate(self, batch, comp, sigma):
+        seen["w"] = batch.weight.copy()
 
-    class FakeBatch:
-        def __init__(self, hb, device):
-            self.host = hb
-
-    monkeypatch.setattr(engine, "DeviceBatch", FakeBatch)
-    monkeypatch.setattr(engine, "spectrum", fake_spectrum)
+    monkeypatch.setattr(SynchRad, "_integrate", fake_integrate)
     monkeypatch.setattr(SynchRad, "_init_raditaion", lambda self, comp, n: setattr(self, "_rad_keys", []))
     monkeypatch.setattr(SynchRad, "_spectr_from_device", lambda self, n: None)
     calc._rad_device = {}
@@ -244,22 +238,28 @@ def test_bench_refuses_more_than_one_gpu():
     assert "one GPU" in msg["error"]
 
 
-def test_autotuner_leaves_large_far_launches_to_the_library_plan():
-    """Far-field launches of >= 1e12 evaluations are ordered and sliced by the
-    library, and no tuning sample ranks their plans; smaller launches and the
-    near field are tuned."""
+def test_autotune_candidates_and_direction_subset_rule():
+    """Candidates keep at least three quarters of a block's lanes busy; large
+    far-field launches (>= 1e12 evaluations) are sampled on a subset of the
+    directions, smaller ones on truncated tracks."""
     from types import SimpleNamespace
     from synchrad_amd import autotune
 
-    def case(mode, n_grid, intervals):
+    split = autotune.candidates("far", "float", True, 512)
+    assert {(c.sweep, c.omega_per_lane, c.waves_per_block) for c in split} == \
+        {("recurrence", 2, 1), ("recurrence", 2, 2), ("recurrence", 2, 4), ("recurrence", 4, 1),
+         ("recurrence", 4, 2)}
+    assert all(c.sweep == "direct" for c in autotune.candidates("far", "float", False, 256))
+    assert {c.omega_per_lane for c in autotune.candidates("far", "double", True, 512)} == {1, 2}
+
+    def large(mode, n_grid, intervals):
         grid = SimpleNamespace(mode=mode, n_grid=n_grid)
         batch = SimpleNamespace(host=SimpleNamespace(n_intervals=lambda: intervals))
-        return autotune.representative(batch, grid)
+        return autotune.is_large(batch, grid)
 
-    assert not case("far", 512 * 64 * 64, 2000 * 7999)     # the betatron beam, 3.4e13
-    assert case("far", 256 * 32 * 32, 1599)                # one undulator electron, 4.2e8
-    assert case("near", 128 * 64 * 64, 500 * 3999)         # near field: not reordered
-    assert case("far", 512 * 64 * 64, 100) and not case("far", 512 * 64 * 64, 500_000)
+    assert large("far", 512 * 64 * 64, 2000 * 7999)         # the betatron beam, 3.4e13
+    assert not large("far", 256 * 32 * 32, 1599)            # one undulator electron, 4.2e8
+    assert not large("near", 128 * 64 * 64, 500 * 3999)     # near field: tiles are not reordered
 
 
 def _polar_spec(mode, comp="total", n=(6, 9, 8), seed=5):

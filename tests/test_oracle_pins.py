@@ -492,3 +492,31 @@ def _safe(pin, L):
         return pin(L)
     except (AssertionError, FloatingPointError, ValueError):
         return False
+
+
+def test_cartesian_screen_oracle_against_the_polar_one():
+    """The Cartesian near-field screen (an extension without a reference
+    counterpart) shares its point body with the polar one: observers on the
+    x axis, (r, 0, L), equal the polar screen's phi = 0 observers
+    (r cos 0, r sin 0, L) bit for bit, each observer's spectrum sits at its
+    own (x, y) index, and an orbit in the x-z plane radiates the same to +y
+    and -y."""
+    n = 600
+    arrs = workloads.planar_undulator(150.0, 1.5, 0.02, n, 0.02 / 40, stagger=False)
+    tr = [list(arrs) + [1.0, 0]]
+    dt = 0.02 / 40
+    w1 = workloads.undulator_omega1(150.0, 1.5, 0.02)
+    omega = np.linspace(0.8, 1.2, 9) * w1
+    r = np.linspace(0.0, 0.05, 5)
+    polar = oracle.spectrum("near", "cartesian", tr, {"omega": omega, "radius": r, "phi": np.array([0.0, 1.0])},
+                            dt, L_screen=7.0)
+    ys = np.array([-0.03, 0.0, 0.03])
+    cart = oracle.spectrum("near", "cartesian", tr, {"omega": omega, "x": r, "y": ys}, dt, L_screen=7.0)
+    for k in ("x", "y", "z"):
+        assert np.array_equal(cart[k][..., 1], polar[k][..., 0])
+        np.testing.assert_allclose(cart[k][..., 0], cart[k][..., 2], rtol=1e-12, atol=1e-300)
+    assert not np.allclose(cart["x"][..., 0], cart["x"][..., 1])
+    # single observer: index (x_i, y_j) holds the spectrum of observer (x_i, y_j, L)
+    one = oracle.spectrum("near", "total", tr, {"omega": omega, "x": r[3:4], "y": ys[2:3]}, dt, L_screen=7.0)
+    full = oracle.spectrum("near", "total", tr, {"omega": omega, "x": r, "y": ys}, dt, L_screen=7.0)
+    assert np.array_equal(one["total"][..., 0, 0], full["total"][..., 3, 2])
