@@ -47,3 +47,29 @@ def test_reductions_on_device_match_host(mode, comp):
     calc.analysis_on_device = True
     calc.Data["radiation"][key] = calc.Data["radiation"][key] * 2.0
     assert not torch.is_tensor(calc._slice(key, -1))
+
+
+@pytest.mark.parametrize("dtype", ["double", "float"])
+def test_repeated_calls_replay_the_captured_launch(dtype):
+    """Repeated calculate_spectrum calls with the same sizes refill the device
+    batch and replay a captured graph; each result equals a fresh calculator's
+    direct call on the same tracks, bit for bit, and the analysis helpers read
+    the replayed spectra."""
+    from synchrad.calc import SynchRad
+    calc = SynchRad(dict(workloads.config2(n_particles=2, n_steps=900, grid=(64, 8, 8)).calc_input, dtype=dtype))
+    for seed in range(4):
+        wl = workloads.config2(n_particles=3, n_steps=900, grid=(64, 8, 8))
+        tracks = wl.track_list()
+        rng = np.random.default_rng(seed)
+        for t in tracks:
+            t[6] = float(rng.uniform(0.5, 2.0))
+            t[0] = t[0] + 1e-7 * rng.standard_normal(t[0].size)
+        calc.calculate_spectrum([list(t) for t in tracks], timeStep=wl.batch.dt, comp="cartesian", nSnaps=2,
+                                verbose=False)
+        fresh = SynchRad(dict(wl.calc_input, dtype=dtype))
+        fresh.calculate_spectrum([list(t) for t in tracks], timeStep=wl.batch.dt, comp="cartesian", nSnaps=2,
+                                 verbose=False)
+        for k in ("x", "y", "z"):
+            assert np.array_equal(calc.Data["radiation"][k], fresh.Data["radiation"][k]), (seed, k)
+        assert calc.get_energy() == pytest.approx(fresh.get_energy(), rel=1e-13)
+        assert (calc._replay["graph"] is not None) == (seed >= 1)

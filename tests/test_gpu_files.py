@@ -95,7 +95,7 @@ def test_nan_gapped_histories_to_spectrum(tmp_path, dtype, tol):
     hist[6, 3, 40:45] = np.nan
     path = str(tmp_path / "pieces.h5")
     pieces = converters.tracks_from_arrays(*hist, cdt=dt, fname=path, shortest_track=8)
-    assert pieces.n_tracks == 7
+    assert pieces.n_tracks == 6            # 2 + 1 + 1 + 2 runs of present samples
     axes = oracle.make_axes([(100.0, 4e4), (0.0, 0.03), (0.0, 2 * np.pi), (40, 6, 5)], "far")
     grid = [(100.0, 4e4), (0.0, 0.03), (0.0, 2 * np.pi), (40, 6, 5)]
     calc = SynchRad({"grid": grid, "dtype": dtype})
