@@ -5,13 +5,15 @@ Default workload (configuration 2 of the benchmark set): the betatron beam,
 2000 electrons x 8000 steps, far field 'total', 512 omega x 64 theta x 64 phi
 = 3.355e13 point-step evaluations per spectrum, split precision.  One "step"
 is one full spectrum of that batch with tracks and grid resident in HBM.  The
-FP64 path is timed on a 200-particle share of the same beam (evals/s does not
-depend on the particle count) and the split result of that share is compared
-with the FP64 one (relative L2).
+FP64 path is timed on the same full batch (one untimed and one timed spectrum)
+and the split spectrum is compared with the FP64 one (relative L2, largest
+error).
 
-Prints one JSON line.  ``roofline`` prices the tile kernel at the reference
-integrand's algorithmic cost (49 flop per far-field evaluation, 54 near)
-against MI355X's FP32 vector peak; ``cpu_baseline`` times the CPU FP64 oracle
+Prints one JSON line.  ``roofline`` sets the tile kernel's executed vector
+work (issued FP32 and FP64 lane-flops per launch, from the rocprofv3 counter
+totals committed in profiles/) against MI355X's vector peaks, and states the
+reference integrand's algorithmic rate (49 flop per far-field evaluation, 54
+near) beside it; ``cpu_baseline`` times the CPU FP64 oracle
 (a C restatement of the reference kernels, not the reference) on a bounded
 share of the same workload on this host, and ``accuracy_vs_oracle`` compares
 both GPU paths with that oracle result (relative L2 and largest error).
@@ -43,7 +45,10 @@ METRIC = "G spectral-pt×step evals/s: split-FP32 vs FP64 @≤1e-4 relL2; % vect
 FLOP_PER_EVAL = {"far": 49.0, "near": 54.0}
 PEAK_FP32_TFLOPS = 157.3
 PEAK_FP64_TFLOPS = 78.6
+N_SIMD = 1024               # 256 CUs x 4 SIMDs
+N_SHADER_ENGINES = 32      # 8 XCDs x 4
 TRAFFIC_FILE = os.path.join(REPO, "profiles", "tile_kernel_betatron_traffic.md")
+COUNTERS_FILE = os.path.join(REPO, "profiles", "tile_kernel_betatron_counters.md")
 
 
 def log(msg):
@@ -57,21 +62,25 @@ def parse():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--config", type=int, default=2, help="workload number (1-5), default 2")
     ap.add_argument("--particles", type=int, default=None, help="override the particle count")
-    ap.add_argument("--f64-particles", type=int, default=200)
+    ap.add_argument("--f64-particles", type=int, default=None,
+                    help="particles of the FP64 leg (default: the whole workload)")
     ap.add_argument("--no-f64", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--autotune", action="store_true",
-                    help="always time candidate plans on a sample first")
     ap.add_argument("--no-autotune", action="store_true", help="always the library's own plan")
     ap.add_argument("--no-e2e", action="store_true", help="skip the drop-in call from host arrays")
     ap.add_argument("--cpu-particles", type=int, default=4)
     ap.add_argument("--cpu-grid", type=str, default="128,32,16")
     ap.add_argument("--per-particle-share", type=int, default=400,
                     help="config 3: particles timed with one launch each")
+    ap.add_argument("--polar-screen", action="store_true",
+                    help="config 4: the reference's polar screen instead of the 64 x 64 Cartesian one")
     ap.add_argument("--copy-calibration", type=int, default=0)
     ap.add_argument("--active-particles", type=int, default=100,
                     help="particles over which the filter active fraction is counted")
     return ap.parse_args()
+
+
+SCREEN = "cartesian"
 
 
 def build_workload(cfg: int, n_particles=None):
@@ -79,16 +88,14 @@ def build_workload(cfg: int, n_particles=None):
     fn = workloads.CONFIGS[cfg]
     if cfg == 1:
         return fn()
+    if cfg == 4:
+        return fn(n_particles=n_particles, screen=SCREEN)
     return fn(n_particles=n_particles)
 
 
 def device_objects(wl):
     from synchrad_amd import engine
-    ax = wl.axes
-    ax1 = ax["theta"] if wl.mode == "far" else ax["radius"]
-    grid = engine.DeviceGrid(wl.mode, ax["omega"], ax1, ax["phi"], L_screen=wl.L_screen)
-    batch = engine.DeviceBatch(wl.batch)
-    return batch, grid
+    return engine.DeviceBatch(wl.batch), wl.device_grid()
 
 
 def time_spectra(batch, grid, comp, precision, steps, warmup, tuning=None):
@@ -176,6 +183,53 @@ def traffic_from_profile():
     return float(m.group(1)) if m else None
 
 
+def counters_from_profile(kind):
+    """Per-dispatch counter totals of the betatron tile kernel ('split' or
+    'f64'), as rocprofv3 --pmc recorded them in profiles/: a line
+    ``counters[kind]: NAME=value NAME=value ...``."""
+    try:
+        text = open(COUNTERS_FILE).read()
+    except OSError:
+        return None
+    m = re.search(r"counters\[" + kind + r"\]:([^\n]*)", text)
+    if not m:
+        return None
+    return {k: float(v) for k, v in re.findall(r"(\w+)=([0-9.eE+-]+)", m.group(1))}
+
+
+def executed_roofline(kind, kernel_ms):
+    """The tile kernel's executed vector work against the MI355X vector peaks.
+
+    Issued lane-flops per launch from the counter totals (a wave instruction
+    is 64 lanes; FMA = 2, add, mul and transcendental = 1; FP32 and FP64 kept
+    apart), divided by the launch time measured now.  ``frac`` is the share of
+    the launch the vector ALUs would need at their peaks for that work, FP32
+    at 157.3 TFLOP/s plus FP64 at 78.6 -- achieved is stated in FP32-equivalent
+    TFLOP/s (an FP64 flop counted as two, the ratio of the peaks)."""
+    c = counters_from_profile(kind)
+    if not c:
+        return None
+    t = kernel_ms * 1e-3
+    f32 = 64.0 * (2 * c.get("SQ_INSTS_VALU_FMA_F32", 0) + c.get("SQ_INSTS_VALU_ADD_F32", 0)
+                  + c.get("SQ_INSTS_VALU_MUL_F32", 0) + c.get("SQ_INSTS_VALU_TRANS_F32", 0))
+    f64 = 64.0 * (2 * c.get("SQ_INSTS_VALU_FMA_F64", 0) + c.get("SQ_INSTS_VALU_ADD_F64", 0)
+                  + c.get("SQ_INSTS_VALU_MUL_F64", 0) + c.get("SQ_INSTS_VALU_TRANS_F64", 0))
+    a32, a64 = f32 / t / 1e12, f64 / t / 1e12
+    out = {"fp32_tflops": round(a32, 3), "fp64_tflops": round(a64, 3),
+           "frac_fp32": round(a32 / PEAK_FP32_TFLOPS, 4), "frac_fp64": round(a64 / PEAK_FP64_TFLOPS, 4),
+           "achieved_fp32_equiv": round(a32 + 2 * a64, 3),
+           "frac": round(a32 / PEAK_FP32_TFLOPS + a64 / PEAK_FP64_TFLOPS, 4),
+           "flop_fp32_per_launch": f32, "flop_fp64_per_launch": f64}
+    if "SQ_INSTS_VALU" in c and c.get("SQ_BUSY_CYCLES", 0) > 0:
+        # a wave64 VALU instruction holds its SIMD for 4 cycles; SQ_BUSY_CYCLES
+        # sums the shader engines' clocks
+        simd_cycles = N_SIMD * c["SQ_BUSY_CYCLES"] / N_SHADER_ENGINES
+        out["valu_issue_share"] = round(4.0 * c["SQ_INSTS_VALU"] / simd_cycles, 4)
+    if "VALUBusy" in c:
+        out["valu_busy_rocprof_pct"] = c["VALUBusy"]
+    return out
+
+
 def cpu_baseline(wl_full, cfg, n_particles, grid_str):
     """FP64 oracle on this host: `n_particles` of the workload on a reduced grid."""
     from oracle import oracle
@@ -213,8 +267,7 @@ def accuracy_vs_oracle(sample):
     from synchrad_amd import engine
     wl, tr, ref, desc = sample
     ax = wl.axes
-    ax1 = ax["theta"] if wl.mode == "far" else ax["radius"]
-    grid = engine.DeviceGrid(wl.mode, ax["omega"], ax1, ax["phi"], L_screen=wl.L_screen)
+    grid = wl.device_grid()
     batch = engine.DeviceBatch(engine.pack_tracks(tr, dt=wl.batch.dt))
     keys = OUT_KEYS[wl.comp]
     r = np.stack([ref[k] for k in keys])
@@ -229,31 +282,36 @@ def accuracy_vs_oracle(sample):
     return out
 
 
-def end_to_end(wl, precision):
-    """One drop-in SynchRad.calculate_spectrum from host track lists: CSR
-    packing, uploads, the library call and the download of the spectrum."""
+def end_to_end(wl, precision, calls=3):
+    """Drop-in SynchRad.calculate_spectrum calls from host track lists: CSR
+    packing, one upload, the library call and the download of the spectrum.
+    The first call launches directly; a repeat with the same sizes refills the
+    device batch and (from the second call) replays a captured HIP graph."""
     import torch
     from synchrad_amd.calc import SynchRad
+    if wl.screen != "polar":
+        return {"skipped": "the drop-in SynchRad API keeps the reference's polar screen only"}
     calc = SynchRad(dict(wl.calc_input, dtype=precision))
     tracks = wl.track_list()
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    calc.calculate_spectrum(particleTracks=tracks, timeStep=wl.batch.dt, comp=wl.comp, L_screen=wl.L_screen,
-                            verbose=False)
-    torch.cuda.synchronize()
-    ms = (time.perf_counter() - t0) * 1e3
-    return {"value": round(wl.evals / (ms * 1e-3) / 1e9, 3), "unit": "G evals/s", "ms": round(ms, 1),
-            "path": "SynchRad.calculate_spectrum from host track lists (CSR packing, H2D, library call, "
-                    "D2H), library's own plan"}
+    times = []
+    for _ in range(calls):
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        calc.calculate_spectrum(particleTracks=tracks, timeStep=wl.batch.dt, comp=wl.comp, L_screen=wl.L_screen,
+                                verbose=False)
+        torch.cuda.synchronize()
+        times.append((time.perf_counter() - t0) * 1e3)
+    steady = times[-1]
+    return {"value": round(wl.evals / (steady * 1e-3) / 1e9, 3), "unit": "G evals/s", "ms": round(steady, 2),
+            "ms_first_call": round(times[0], 2), "ms_second_call_with_capture": round(times[1], 2),
+            "path": "SynchRad.calculate_spectrum from host track lists (CSR packing, one H2D copy, library "
+                    "call, D2H); 'ms' is a repeated call of the same sizes, which replays the captured graph"}
 
 
 def plan_for(batch, grid, comp, precision, args):
     from dataclasses import asdict
     from synchrad_amd import autotune
     if args.no_autotune:
-        return None, None
-    if not args.autotune and not autotune.representative(batch, grid):
-        # the autotuner's sample would run in another regime than the launch
         return None, None
     t0 = time.perf_counter()
     plan, rows = autotune.tune(batch, grid, comp=comp, precision=precision)
@@ -279,7 +337,6 @@ def bench_forward(args):
     ms, ms_kernel, wall, outs = time_spectra(batch, grid, wl.comp, main_prec, args.steps, args.warmup, plan)
     value = evals / (ms * 1e-3) / 1e9
     achieved_tf = evals * flop / (ms_kernel * 1e-3) / 1e12
-    peak = PEAK_FP64_TFLOPS if main_prec == "double" else PEAK_FP32_TFLOPS
     log(f"{main_prec}: {ms:.1f} ms/spectrum ({ms_kernel:.1f} ms in the library call, {wall:.1f} ms wall), "
         f"{value:.1f} G evals/s, {achieved_tf:.1f} TFLOP/s algorithmic")
     nmax = int(np.diff(wl.batch.offsets).max())
@@ -305,33 +362,25 @@ def bench_forward(args):
                    "evals_per_spectrum": evals,
                    "precision": "FP64" if main_prec == "double" else "split-FP32",
                    "plan": plan_d},
-        "roofline": {"bound": "valu", "achieved": round(achieved_tf, 3), "peak": peak,
-                     "unit": "TFLOP/s", "frac": round(achieved_tf / peak, 4),
-                     "traffic": traffic_from_profile() if cfg == 2 else None,
-                     "kernel_ms": round(ms_kernel, 3),
-                     "filter_active_fraction": round(f_act, 4) if f_act is not None else None,
-                     "frac_filter_aware": (round(evals * (9.0 + 40.0 * f_act) / (ms_kernel * 1e-3) / 1e12 / peak, 4)
-                                           if (f_act is not None and flop == 49.0) else None),
-                     "basis": f"{flop:g} algorithmic flop per point-step evaluation x "
-                              f"{evals:.4g} evaluations per launch / mean duration of the library "
-                              f"call (HIP events on its stream); frac_filter_aware counts 9 flop "
-                              f"for an evaluation the filter rejects and 49 for one it keeps"},
+        "roofline": roofline_entry(cfg, main_prec, evals, flop, ms_kernel, f_act),
     }
 
     if main_prec == "float" and not args.no_f64:
-        n64 = min(args.f64_particles, wl.batch.n_particles)
+        n64 = wl.batch.n_particles if args.f64_particles is None else min(args.f64_particles,
+                                                                          wl.batch.n_particles)
         wl64 = build_workload(cfg, n64) if n64 < wl.batch.n_particles else wl
         b64, g64 = (batch, grid) if wl64 is wl else device_objects(wl64)
-        log(f"timing FP64 on {n64} particles")
-        ms64, _, _, outs64 = time_spectra(b64, g64, wl.comp, "double", 1, 0)
+        log(f"timing FP64 on {n64} particles (1 untimed + 1 timed spectrum)")
+        ms64, ms64_kernel, _, outs64 = time_spectra(b64, g64, wl.comp, "double", 1, 1)
         ev64 = wl64.evals
         v64 = ev64 / (ms64 * 1e-3) / 1e9
-        _, _, _, outs_s = time_spectra(b64, g64, wl.comp, "float", 1, 0, plan)
+        outs_s = outs if wl64 is wl else time_spectra(b64, g64, wl.comp, "float", 1, 0, plan)[3]
         rel = rel_l2_dev(outs_s, outs64)
         mx = max_err_dev(outs_s, outs64)
         log(f"FP64: {ms64:.1f} ms ({v64:.1f} G evals/s); split vs FP64 rel-L2 {rel:.3e}")
-        res["f64"] = {"value": round(v64, 3), "unit": "G evals/s", "particles": n64,
-                      "frac_fp64_peak": round(v64 * 1e9 * flop / 1e12 / PEAK_FP64_TFLOPS, 4),
+        res["f64"] = {"value": round(v64, 3), "unit": "G evals/s", "particles": n64, "ms": round(ms64, 3),
+                      "roofline": roofline_entry(cfg if wl64 is wl else None, "double", ev64, flop, ms64_kernel,
+                                                 None),
                       "form": "this library's FP64 path: FP64 phase, exact filter, FP64 table sin/cos, "
                               "FP64 sums (the same staged tiles as the split path)"}
         res["split_over_f64"] = round(value / v64, 3)
@@ -353,6 +402,35 @@ def bench_forward(args):
         res["accuracy_vs_oracle"] = accuracy_vs_oracle(sample)
         log(f"accuracy: {res['accuracy_vs_oracle']}")
     return res
+
+
+def roofline_entry(cfg, precision, evals, flop, kernel_ms, f_act):
+    """The dominant kernel against the vector ALU roofline.  ``achieved`` /
+    ``frac``: the executed vector work from the counter totals in profiles/
+    (betatron workload only; see executed_roofline).  ``algorithmic``: the
+    reference integrand's flop count per evaluation times the evaluations,
+    which exceeds the peak because this kernel shares the omega-independent
+    work, rotates instead of taking every sin/cos and skips filtered steps."""
+    kind = "f64" if precision == "double" else "split"
+    ex = executed_roofline(kind, kernel_ms) if cfg == 2 else None
+    alg = evals * flop / (kernel_ms * 1e-3) / 1e12
+    peak_alg = PEAK_FP64_TFLOPS if precision == "double" else PEAK_FP32_TFLOPS
+    out = {"bound": "valu",
+           "achieved": ex["achieved_fp32_equiv"] if ex else None,
+           "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
+           "frac": ex["frac"] if ex else None,
+           "traffic": traffic_from_profile() if (cfg == 2 and kind == "split") else None,
+           "kernel_ms": round(kernel_ms, 3),
+           "executed": ex,
+           "algorithmic": {"flop_per_eval": flop, "tflops": round(alg, 3), "peak": peak_alg,
+                           "rate_over_peak": round(alg / peak_alg, 4),
+                           "filter_active_fraction": round(f_act, 4) if f_act is not None else None},
+           "basis": "achieved/frac: issued lane-flops per launch from rocprofv3 counter totals "
+                    "(profiles/tile_kernel_betatron_counters.md; FMA 2, add/mul/transcendental 1) over "
+                    "this run's launch time (HIP events on the launch stream), FP64 counted as two "
+                    "FP32 flops; algorithmic: the reference integrand's flop count per point-step "
+                    "evaluation, a rate, not a hardware fraction"}
+    return out
 
 
 def per_particle_compare(wl, grid, plan, n_share):
@@ -450,7 +528,10 @@ def copy_calibration(nbytes):
 
 
 def main():
+    global SCREEN
     args = parse()
+    if args.polar_screen:
+        SCREEN = "polar"
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1 (multi-GPU sharding "
                                    "is designed but not built)"}))
