@@ -45,8 +45,6 @@ METRIC = "G spectral-pt×step evals/s: split-FP32 vs FP64 @≤1e-4 relL2; % vect
 FLOP_PER_EVAL = {"far": 49.0, "near": 54.0}
 PEAK_FP32_TFLOPS = 157.3
 PEAK_FP64_TFLOPS = 78.6
-N_SIMD = 1024               # 256 CUs x 4 SIMDs
-N_SHADER_ENGINES = 32      # 8 XCDs x 4
 TRAFFIC_FILE = os.path.join(REPO, "profiles", "tile_kernel_betatron_traffic.md")
 COUNTERS_FILE = os.path.join(REPO, "profiles", "tile_kernel_betatron_counters.md")
 
@@ -220,11 +218,6 @@ def executed_roofline(kind, kernel_ms):
            "achieved_fp32_equiv": round(a32 + 2 * a64, 3),
            "frac": round(a32 / PEAK_FP32_TFLOPS + a64 / PEAK_FP64_TFLOPS, 4),
            "flop_fp32_per_launch": f32, "flop_fp64_per_launch": f64}
-    if "SQ_INSTS_VALU" in c and c.get("SQ_BUSY_CYCLES", 0) > 0:
-        # a wave64 VALU instruction holds its SIMD for 4 cycles; SQ_BUSY_CYCLES
-        # sums the shader engines' clocks
-        simd_cycles = N_SIMD * c["SQ_BUSY_CYCLES"] / N_SHADER_ENGINES
-        out["valu_issue_share"] = round(4.0 * c["SQ_INSTS_VALU"] / simd_cycles, 4)
     if "VALUBusy" in c:
         out["valu_busy_rocprof_pct"] = c["VALUBusy"]
     return out
@@ -243,8 +236,7 @@ def cpu_baseline(wl_full, cfg, n_particles, grid_str):
         sample_grid = "x".join(str(v) for v in wl.calc_input["grid"][-1])
     tr = wl.track_list()[:n_particles]
     ax = wl.axes
-    n1 = (ax["theta"] if wl.mode == "far" else ax["radius"]).size
-    evals = ax["omega"].size * n1 * ax["phi"].size * sum(max(t[0].size - 1, 0) for t in tr)
+    evals = wl.n_grid * sum(max(t[0].size - 1, 0) for t in tr)
     threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
     t0 = time.perf_counter()
     ref = oracle.spectrum(wl.mode, wl.comp, tr, ax, wl.batch.dt, L_screen=wl.L_screen, threads=threads)

@@ -58,6 +58,36 @@ __device__ __forceinline__ int64_t uni64(int64_t v) {
 __device__ __forceinline__ double unid(double v) {
   return __longlong_as_double(uni64(__double_as_longlong(v)));
 }
+// The value unchanged, but opaque to the optimiser from this point: addresses
+// formed from it are formed here, not hoisted out of the loops around (where
+// the compiler would keep one 64-bit per-lane pointer per output and spill
+// them across the sweep).  An empty asm statement: it emits no instruction.
+template <typename V>
+__device__ __forceinline__ V late(V v) {
+  asm volatile("" : "+v"(v));
+  return v;
+}
+
+// Division of n < 2^31 by a divisor fixed for a launch, as a multiply and a
+// shift (Granlund and Montgomery, "Division by invariant integers using
+// multiplication", 1994, Thm 4.2 with N = 31): l = ceil(log2 d),
+// m = ceil(2^(31+l) / d) < 2^32 gives floor(n / d) = (n m) >> (31 + l).  With
+// n and the divisor wave-uniform this runs on the scalar unit; the compiler's
+// own division would form its reciprocal in vector registers and hold it
+// across the tile loop.
+struct UDiv {
+  uint32_t m, sh;
+};
+inline UDiv make_udiv(uint32_t d) {
+  if (d == 0) return UDiv{0u, 31u};
+  uint32_t l = 0;
+  while ((1ull << l) < d) l++;
+  const uint64_t m = ((1ull << (31 + l)) + d - 1) / d;
+  return UDiv{(uint32_t)m, 31u + l};
+}
+__host__ __device__ __forceinline__ uint32_t udiv(uint32_t n, UDiv v) {
+  return (uint32_t)(((uint64_t)n * v.m) >> v.sh);
+}
 
 // FP64 sin and cos of a phase.  The quadrant k = rint(2 ph / pi) is removed
 // with pi/2 split in two (hi = fl(pi/2), lo = pi/2 - hi; the FMA forms

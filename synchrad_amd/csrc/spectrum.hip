@@ -136,6 +136,7 @@ struct KParams {
   // tiling
   uint32_t n_chunks;
   uint32_t n_dir;
+  UDiv div_chunks, div_dir, div_theta;   // by n_chunks, n_dir, n_theta (tiles < 2^31)
   int32_t slices;
   int64_t n_tiles;
   int64_t slice_lo[kMaxSlices + 1];
@@ -259,6 +260,9 @@ struct LdsRows {
 #ifndef SR_F64_WAVES2
 #define SR_F64_WAVES2 4
 #endif
+#ifndef SR_F64_NEAR_WAVES2
+#define SR_F64_NEAR_WAVES2 SR_F64_WAVES2
+#endif
 // Recurrence sweep: the steps of a tile are put in an order in which every
 // 64-omega run of the block finds the steps that keep any of its omegas first,
 // so a wave sweeps only those (a stable counting sort per tile).  The direct
@@ -271,7 +275,8 @@ struct LdsRows {
 template <int MODE, int PREC, int SWEEP, int M>
 struct WavesPerEU {
   static constexpr int value =
-      (PREC == SR_PREC_F64) ? (M == 2 ? SR_F64_WAVES2 : SR_F64_WAVES)
+      (PREC == SR_PREC_F64) ? (M == 2 ? (MODE == SR_MODE_NEAR ? SR_F64_NEAR_WAVES2 : SR_F64_WAVES2)
+                                      : SR_F64_WAVES)
       : (SWEEP == SWEEP_REC) ? ((MODE == SR_MODE_FAR && M == 2)   ? SR_REC_WAVES_FAR2
                                 : (MODE == SR_MODE_FAR && M == 4) ? SR_REC_WAVES_FAR4
                                                                   : SR_REC_WAVES)
@@ -348,12 +353,13 @@ k_spectrum(const KParams p) {
   const double rot_rev = (double)T * p.omega_step * kInv2Pi;   // revolutions of the k -> k+1 step
 
   for (int64_t ti = blockIdx.x; ti < p.n_tiles; ti += gridDim.x) {
-    const int64_t tile = p.tile_order ? (int64_t)p.tile_order[ti] : ti;
-    const uint32_t chunk = (uint32_t)(tile % p.n_chunks);
-    const int64_t rest = tile / p.n_chunks;
-    const uint32_t dir = (uint32_t)(rest % p.n_dir);
-    const int q = (int)(rest / p.n_dir);
-    const uint32_t iPhi = dir / p.n_theta;
+    const uint32_t tile = p.tile_order ? (uint32_t)p.tile_order[ti] : (uint32_t)ti;
+    const uint32_t rest = udiv(tile, p.div_chunks);
+    const uint32_t chunk = tile - rest * p.n_chunks;
+    const uint32_t qs = udiv(rest, p.div_dir);
+    const uint32_t dir = rest - qs * p.n_dir;
+    const int q = (int)qs;
+    const uint32_t iPhi = udiv(dir, p.div_theta);
     const uint32_t iTh = dir - iPhi * p.n_theta;
 
     // geometry of the block's point (kernel_farfield.cl:40-42, 384-387;
@@ -367,13 +373,14 @@ k_spectrum(const KParams p) {
     };
     double ngx = 0.0, ngy = 0.0, ngz = 0.0;
     if (MODE == SR_MODE_NEAR) {
+      // the block's screen point is wave-uniform: kept in scalar registers
       const double r = p.radius[iTh];
       if (p.screen_y) {
-        ngx = r;
-        ngy = p.screen_y[iPhi];
+        ngx = unid(r);
+        ngy = unid(p.screen_y[iPhi]);
       } else {
-        ngx = r * p.cos_p[iPhi];
-        ngy = r * p.sin_p[iPhi];
+        ngx = unid(r * p.cos_p[iPhi]);
+        ngy = unid(r * p.sin_p[iPhi]);
       }
       ngz = p.l_screen;
     }
@@ -395,7 +402,7 @@ k_spectrum(const KParams p) {
     // the block's omegas are dealt to its waves in 64-omega runs; the low runs
     // keep the most steps, so the run -> wave assignment is rotated by the tile
     // index to spread the heavy waves of co-resident blocks over the SIMDs
-    const int vw = SR_WAVE_ROTATE ? (int)(((int64_t)wv + tile) % W) : wv;
+    const int vw = SR_WAVE_ROTATE ? (int)((wv + tile) & (uint32_t)(W - 1)) : wv;   // W is 1, 2 or 4
     // the omega index of the thread's k-th omega: lane of run k*W + vw
     auto omega_slot = [&](int k) { return (int)(chunk * T * M + (k * W + vw) * kWave + lane); };
     int wi[M];
@@ -556,7 +563,7 @@ k_spectrum(const KParams p) {
         if (p.acc_lds) {
           s_acc[k * T] += v[0];
         } else {
-          const int64_t g = (int64_t)slot * p.n_grid + (int64_t)dir * p.n_omega + wk +
+          const int64_t g = (int64_t)slot * p.n_grid + (int64_t)dir * p.n_omega + late(wk) +
                             (ip - p.part_base) * p.part_stride;
 #pragma unroll
           for (int c = 0; c < kMaxOut; c++)
@@ -1424,6 +1431,11 @@ int run(KParams &kp, const Plan &pl, const sr_grid &g, const sr_opts &o, void *w
   kp.n_chunks = (uint32_t)((g.n_omega + run - 1) / run);
   kp.slices = pl.slices;
   kp.n_tiles = pl.blocks;
+  if (pl.blocks >= ((int64_t)1 << 31))
+    return fail(SR_ERR_UNSUPPORTED, "%lld tiles in one launch: at most 2^31 - 1", (long long)pl.blocks);
+  kp.div_chunks = make_udiv(kp.n_chunks);
+  kp.div_dir = make_udiv(kp.n_dir);
+  kp.div_theta = make_udiv(kp.n_theta);
   kp.acc_lds = (kp.n_out == 1 && kp.n_snaps == 1) ? 1 : 0;
   // per-particle outputs are disjoint: slices write them in place
   const size_t need = kp.part_stride ? 0 : scratch_bytes(pl, kp.n_out, kp.n_grid, kp.n_snaps);

@@ -153,13 +153,14 @@ def pack_tracks(tracks: Sequence[Sequence], *, dt: float, it_range=None, nSnaps:
 _PACK_ALIGN = 256
 
 
-def _packed_layout(hb: HostBatch):
+def _packed_layout(hb: HostBatch, with_tracks: bool = True):
     """Byte offsets of a batch's arrays in one contiguous buffer: offsets
-    (int64), the six coordinate and momentum arrays and the weights (float64),
-    it_start, it_end and it_snaps (uint32), each 256-byte aligned."""
-    parts = [("offsets", hb.offsets.size * 8)] + [(c, hb.n_points * 8) for c in ("x", "y", "z", "ux", "uy", "uz")] + \
-        [("weight", hb.n_particles * 8), ("it_start", hb.n_particles * 4), ("it_end", hb.n_particles * 4),
-         ("it_snaps", hb.it_snaps.size * 4)]
+    (int64), the six coordinate and momentum arrays and the weights (float64;
+    left out when the caller supplies them as device tensors), it_start,
+    it_end and it_snaps (uint32), each 256-byte aligned."""
+    tracks = [(c, hb.n_points * 8) for c in ("x", "y", "z", "ux", "uy", "uz")] + [("weight", hb.n_particles * 8)]
+    parts = [("offsets", hb.offsets.size * 8)] + (tracks if with_tracks else []) + \
+        [("it_start", hb.n_particles * 4), ("it_end", hb.n_particles * 4), ("it_snaps", hb.it_snaps.size * 4)]
     layout, at = {}, 0
     for name, nbytes in parts:
         layout[name] = (at, nbytes)
@@ -194,7 +195,8 @@ class DeviceBatch:
         dev = _dev(device)
         self.host = hb
         self.device = dev
-        self._layout, total = _packed_layout(hb)
+        self._with_tracks = tensors is None
+        self._layout, total = _packed_layout(hb, self._with_tracks)
         self._buf = torch.from_numpy(_pack_host(hb, self._layout, total)).to(dev)
         views = {}
         for name, (at, nbytes) in self._layout.items():
@@ -229,7 +231,7 @@ class DeviceBatch:
                                 int(old.it_snaps.size), old.dt):
             self.host = old
             raise ValueError("refill needs a batch of the same sizes, snapshot form and time step")
-        layout, total = _packed_layout(hb)
+        layout, total = _packed_layout(hb, self._with_tracks)
         self._buf.copy_(torch.from_numpy(_pack_host(hb, layout, total)))
         self._offsets_host = np.ascontiguousarray(hb.offsets, dtype=np.int64)
 

@@ -354,3 +354,20 @@ def test_coherent_components():
     y = s.get_full_spectrum(comp="y", iteration=0)
     assert np.iscomplexobj(y)
     np.testing.assert_allclose(y, pref * (rad["yre"][0] + 1j * rad["yim"][0]), rtol=1e-14)
+
+
+@pytest.mark.parametrize("screen", ["polar", "cartesian"])
+def test_bench_cpu_baseline_counts_the_sample(screen):
+    """bench.py's CPU leg counts evaluations as grid points x step intervals of
+    its sample, on either near-field screen."""
+    import importlib.util
+    import os
+    spec = importlib.util.spec_from_file_location(
+        "bench_mod", os.path.join(os.path.dirname(os.path.dirname(__file__)), "bench.py"))
+    bench = importlib.util.module_from_spec(spec)
+    spec.loader.exec_module(bench)
+    wl = workloads.config4(scale=0.125, n_particles=1, screen=screen)
+    res, sample = bench.cpu_baseline(wl, 4, 1, "0,0,0")
+    assert wl.n_grid == 16 * 8 * 8
+    assert res["value"] > 0 and f"{wl.n_grid * 3999:.3g} evals" in res["sample"]
+    assert sample[2]["total"].shape == (1, 16, 8, 8)
