@@ -47,6 +47,9 @@ PEAK_FP32_TFLOPS = 157.3
 PEAK_FP64_TFLOPS = 78.6
 TRAFFIC_FILE = os.path.join(REPO, "profiles", "tile_kernel_betatron_traffic.md")
 COUNTERS_FILE = os.path.join(REPO, "profiles", "tile_kernel_betatron_counters.md")
+ADJOINT_COUNTERS_FILE = os.path.join(REPO, "profiles", "adjoint_betatron_counters.md")
+# CPU oracle sample per configuration: ~10-25 s on 16 host cores
+CPU_SAMPLE_PARTICLES = {1: 1, 2: 24, 3: 24, 4: 4}
 
 
 def log(msg):
@@ -66,7 +69,9 @@ def parse():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-autotune", action="store_true", help="always the library's own plan")
     ap.add_argument("--no-e2e", action="store_true", help="skip the drop-in call from host arrays")
-    ap.add_argument("--cpu-particles", type=int, default=4)
+    ap.add_argument("--cpu-particles", type=int, default=None,
+                    help="particles of the CPU oracle sample (default: about 10-25 s of CPU work: "
+                         "24 for configurations 2 and 3, 4 for 4)")
     ap.add_argument("--cpu-grid", type=str, default="128,32,16")
     ap.add_argument("--per-particle-share", type=int, default=400,
                     help="config 3: particles timed with one launch each")
@@ -181,12 +186,13 @@ def traffic_from_profile():
     return float(m.group(1)) if m else None
 
 
-def counters_from_profile(kind):
-    """Per-dispatch counter totals of the betatron tile kernel ('split' or
-    'f64'), as rocprofv3 --pmc recorded them in profiles/: a line
+def counters_from_profile(kind, path=COUNTERS_FILE):
+    """Per-dispatch counter totals of a profiled kernel (the betatron tile
+    kernel: 'split' or 'f64'; the adjoint's transpose sweep: 'adjoint_split'
+    or 'adjoint_f64'), as rocprofv3 --pmc recorded them in profiles/: a line
     ``counters[kind]: NAME=value NAME=value ...``."""
     try:
-        text = open(COUNTERS_FILE).read()
+        text = open(path).read()
     except OSError:
         return None
     m = re.search(r"counters\[" + kind + r"\]:([^\n]*)", text)
@@ -195,7 +201,7 @@ def counters_from_profile(kind):
     return {k: float(v) for k, v in re.findall(r"(\w+)=([0-9.eE+-]+)", m.group(1))}
 
 
-def executed_roofline(kind, kernel_ms):
+def executed_roofline(kind, kernel_ms, path=COUNTERS_FILE):
     """The tile kernel's executed vector work against the MI355X vector peaks.
 
     Issued lane-flops per launch from the counter totals (a wave instruction
@@ -204,7 +210,7 @@ def executed_roofline(kind, kernel_ms):
     the launch the vector ALUs would need at their peaks for that work, FP32
     at 157.3 TFLOP/s plus FP64 at 78.6 -- achieved is stated in FP32-equivalent
     TFLOP/s (an FP64 flop counted as two, the ratio of the peaks)."""
-    c = counters_from_profile(kind)
+    c = counters_from_profile(kind, path)
     if not c:
         return None
     t = kernel_ms * 1e-3
@@ -389,7 +395,8 @@ def bench_forward(args):
 
     if not args.no_cpu:
         log("timing CPU oracle sample")
-        res["cpu_baseline"], sample = cpu_baseline(wl, cfg, args.cpu_particles, args.cpu_grid)
+        n_cpu = args.cpu_particles if args.cpu_particles is not None else CPU_SAMPLE_PARTICLES.get(cfg, 4)
+        res["cpu_baseline"], sample = cpu_baseline(wl, cfg, n_cpu, args.cpu_grid)
         log(f"cpu: {res['cpu_baseline']}")
         res["accuracy_vs_oracle"] = accuracy_vs_oracle(sample)
         log(f"accuracy: {res['accuracy_vs_oracle']}")
@@ -479,29 +486,45 @@ def bench_adjoint(args):
            "higher_is_better": False, "data": "synthetic",
            "config": {"workload": f"adjoint-betatron: {hb.n_particles} e- x 4000 steps (fp32 tracks), "
                                   "far 'total', grid 128x32x32", "evals_forward": evals}}
+    st = torch.cuda.current_stream()
     for prec in ("float", "double"):
-        def step():
+        def step(ev=None):
             for t in leaves:
                 t.grad = None
+            if ev:
+                ev[0].record(st)
             S = autograd.far_spectrum(*leaves, hb, grid, comp="total", precision=prec)[0]
             loss = torch.sum(W * (S - T) ** 2)
+            if ev:
+                ev[1].record(st)
             loss.backward()
+            if ev:
+                ev[2].record(st)
             return loss
         step()
         torch.cuda.synchronize()
-        st = torch.cuda.current_stream()
-        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-        e0.record(st)
-        for _ in range(args.steps):
-            step()
-        e1.record(st)
+        evs = [[torch.cuda.Event(enable_timing=True) for _ in range(3)] for _ in range(args.steps)]
+        for ev in evs:
+            step(ev)
         torch.cuda.synchronize()
-        ms = e0.elapsed_time(e1) / args.steps
-        res[prec] = {"ms_fwd_bwd": round(ms, 2),
-                     "G_evals_per_s_fwd_equiv": round(evals / (ms * 1e-3) / 1e9, 2)}
-        log(f"adjoint {prec}: {ms:.1f} ms forward+backward")
+        ms = sum(ev[0].elapsed_time(ev[2]) for ev in evs) / args.steps
+        ms_bwd = sum(ev[1].elapsed_time(ev[2]) for ev in evs) / args.steps
+        kind = "adjoint_split" if prec == "float" else "adjoint_f64"
+        ex = executed_roofline(kind, ms_bwd, ADJOINT_COUNTERS_FILE)
+        res[prec] = {"ms_fwd_bwd": round(ms, 2), "ms_forward": round(ms - ms_bwd, 2), "ms_backward": round(ms_bwd, 2),
+                     "G_evals_per_s_fwd_equiv": round(evals / (ms * 1e-3) / 1e9, 2),
+                     "roofline_backward": {"bound": "valu", "kernel": "k_adjoint (transpose sweep)",
+                                           "achieved": ex["achieved_fp32_equiv"] if ex else None,
+                                           "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
+                                           "frac": ex["frac"] if ex else None, "executed": ex,
+                                           "basis": "issued lane-flops of k_adjoint per backward from the counter "
+                                                    "totals in profiles/adjoint_betatron_counters.md over this "
+                                                    "run's backward time (HIP events around loss.backward(); "
+                                                    "includes the slab reduction and the loss's own gradient)"}}
+        log(f"adjoint {prec}: {ms:.1f} ms forward+backward ({ms_bwd:.1f} ms backward)")
     res["value"] = res["float"]["ms_fwd_bwd"]
     res["ms_per_step"] = res["value"]
+    res["roofline"] = res["float"]["roofline_backward"]
     res["steps"], res["warmup"] = args.steps, 1
     res["dtype"] = "f32 (f64 phase)"
     return res

@@ -71,6 +71,9 @@ class HostBatch:
     n_snaps: int
     snaps_stride: int
     dt: float
+    # the upload buffer (uint8) when the arrays were written straight into it
+    # (pack_tracks): the device upload then copies the tracks only once
+    packed: Optional[np.ndarray] = field(default=None, repr=False, compare=False)
 
     @property
     def n_particles(self) -> int:
@@ -131,20 +134,28 @@ def pack_csr(offsets, x, y, z, ux, uy, uz, weight, it_start=None, *, dt: float,
 
 def pack_tracks(tracks: Sequence[Sequence], *, dt: float, it_range=None, nSnaps: int = 1) -> HostBatch:
     """Pack reference-style track lists ``[x, y, z, ux, uy, uz, w, it_start]`` (or
-    the 7-item form, synchrad/calc.py:580-584)."""
+    the 7-item form, synchrad/calc.py:580-584).  The six coordinate and
+    momentum columns are concatenated straight into the batch's upload buffer
+    (``HostBatch.packed``), so the tracks are copied once on the host."""
     Np = len(tracks)
     lengths = np.array([np.asarray(t[0]).size for t in tracks], dtype=np.int64)
     offsets = np.zeros(Np + 1, dtype=np.int64)
     np.cumsum(lengths, out=offsets[1:])
+    n_snap_entries = Np * int(nSnaps) if it_range is None else int(nSnaps)
+    layout, total = _layout(Np, int(offsets[-1]), n_snap_entries, True)
+    buf = np.empty(total, dtype=np.uint8)
     cols = []
-    for c in range(6):
+    for c, name in enumerate(("x", "y", "z", "ux", "uy", "uz")):
+        at, nbytes = layout[name]
+        col = buf[at:at + nbytes].view(np.float64)
         if Np:
-            cols.append(np.concatenate([np.asarray(t[c], dtype=np.float64).ravel() for t in tracks]))
-        else:
-            cols.append(np.zeros(0, dtype=np.float64))
+            np.concatenate([np.asarray(t[c], dtype=np.float64).ravel() for t in tracks], out=col)
+        cols.append(col)
     w = np.array([float(t[6]) for t in tracks], dtype=np.float64)
     its = np.array([int(np.uint32(t[7])) if len(t) == 8 else 0 for t in tracks], dtype=np.uint32)
-    return pack_csr(offsets, *cols, w, its, dt=dt, it_range=it_range, nSnaps=nSnaps)
+    hb = pack_csr(offsets, *cols, w, its, dt=dt, it_range=it_range, nSnaps=nSnaps)
+    hb.packed = buf
+    return hb
 
 
 # ---------------------------------------------------------------------------
@@ -153,14 +164,14 @@ def pack_tracks(tracks: Sequence[Sequence], *, dt: float, it_range=None, nSnaps:
 _PACK_ALIGN = 256
 
 
-def _packed_layout(hb: HostBatch, with_tracks: bool = True):
+def _layout(n_particles: int, n_points: int, n_snap_entries: int, with_tracks: bool):
     """Byte offsets of a batch's arrays in one contiguous buffer: offsets
     (int64), the six coordinate and momentum arrays and the weights (float64;
     left out when the caller supplies them as device tensors), it_start,
     it_end and it_snaps (uint32), each 256-byte aligned."""
-    tracks = [(c, hb.n_points * 8) for c in ("x", "y", "z", "ux", "uy", "uz")] + [("weight", hb.n_particles * 8)]
-    parts = [("offsets", hb.offsets.size * 8)] + (tracks if with_tracks else []) + \
-        [("it_start", hb.n_particles * 4), ("it_end", hb.n_particles * 4), ("it_snaps", hb.it_snaps.size * 4)]
+    tracks = [(c, n_points * 8) for c in ("x", "y", "z", "ux", "uy", "uz")] + [("weight", n_particles * 8)]
+    parts = [("offsets", (n_particles + 1) * 8)] + (tracks if with_tracks else []) + \
+        [("it_start", n_particles * 4), ("it_end", n_particles * 4), ("it_snaps", n_snap_entries * 4)]
     layout, at = {}, 0
     for name, nbytes in parts:
         layout[name] = (at, nbytes)
@@ -168,9 +179,19 @@ def _packed_layout(hb: HostBatch, with_tracks: bool = True):
     return layout, max(at, _PACK_ALIGN)
 
 
+def _packed_layout(hb: HostBatch, with_tracks: bool = True):
+    return _layout(hb.n_particles, hb.n_points, int(hb.it_snaps.size), with_tracks)
+
+
 def _pack_host(hb: HostBatch, layout, total) -> np.ndarray:
-    buf = np.empty(total, dtype=np.uint8)
+    """The upload buffer of ``hb``: its own ``packed`` buffer when it has the
+    layout's size (arrays already in place are not copied again), else a new one."""
+    own = hb.packed is not None and hb.packed.size == total
+    buf = hb.packed if own else np.empty(total, dtype=np.uint8)
+    base = buf.ctypes.data
     for name, (at, nbytes) in layout.items():
+        if own and nbytes and getattr(hb, name).ctypes.data == base + at:
+            continue
         if name in ("it_start", "it_end", "it_snaps"):
             arr = np.ascontiguousarray(getattr(hb, name), dtype=np.uint32)
         elif name == "offsets":

@@ -371,3 +371,25 @@ def test_bench_cpu_baseline_counts_the_sample(screen):
     assert wl.n_grid == 16 * 8 * 8
     assert res["value"] > 0 and f"{wl.n_grid * 3999:.3g} evals" in res["sample"]
     assert sample[2]["total"].shape == (1, 16, 8, 8)
+
+
+def test_pack_tracks_writes_the_upload_buffer_once():
+    """pack_tracks concatenates the columns straight into the batch's upload
+    buffer; the upload packing then reuses that buffer and its bytes equal a
+    fresh packing of the same arrays."""
+    rng = np.random.default_rng(5)
+    tracks = [[rng.standard_normal(n) for _ in range(6)] + [float(rng.uniform(0.5, 2)), int(i)]
+              for i, n in enumerate((5, 0, 17, 1, 9))]
+    for kw in ({}, {"it_range": (0, 20), "nSnaps": 3}):
+        hb = engine.pack_tracks(tracks, dt=0.1, **kw)
+        layout, total = engine._packed_layout(hb)
+        buf = engine._pack_host(hb, layout, total)
+        assert buf is hb.packed
+        fresh = engine.pack_csr(hb.offsets.copy(), *(np.array(getattr(hb, c)) for c in ("x", "y", "z", "ux", "uy", "uz")),
+                                hb.weight.copy(), hb.it_start.copy(), dt=0.1, **kw)
+        assert fresh.packed is None
+        other = engine._pack_host(fresh, layout, total)
+        for name, (at, nbytes) in layout.items():
+            assert bytes(buf[at:at + nbytes]) == bytes(other[at:at + nbytes]), name
+        for i, c in enumerate(("x", "y", "z", "ux", "uy", "uz")):
+            np.testing.assert_array_equal(getattr(hb, c), np.concatenate([t[i] for t in tracks]))
