@@ -271,6 +271,14 @@ struct LdsRows {
 #ifndef SR_COMPACT
 #define SR_COMPACT 1
 #endif
+// recurrence sweep: a step's previous retarded time tau_{s-1} comes from the
+// neighbouring thread's stage through LDS (1) instead of being recomputed
+// from the previous sample (0); the tile's first thread recomputes it.  Same
+// results; three global loads fewer per step (the HBM traffic is unchanged:
+// the neighbouring lane's load had brought those lines in)
+#ifndef SR_TAU_EXCHANGE
+#define SR_TAU_EXCHANGE 1
+#endif
 // 1: a step's bucket (the number of runs it keeps) by binary search over the
 // runs' sorted smallest |omega| (log2 R compares; +0.8 % on the betatron
 // launch, same buckets); 0: by comparing with every run
@@ -305,6 +313,9 @@ k_spectrum(const KParams p) {
   constexpr int NA = amp_words<MODE, PREC>();
   constexpr int NC = (MODE == SR_MODE_FAR) ? 2 * NA : 6;   // real accumulators per omega
   constexpr bool kCompact = kSplit && kRec && (SR_COMPACT != 0);
+  // (the direct sweep has no step sort whose barriers it could share: it
+  // recomputes tau_{s-1}, which measured 0.5 % faster there)
+  constexpr bool kExch = kSplit && kRec && (SR_TAU_EXCHANGE != 0);
   constexpr int kMaxW = kMaxT / kWave;                 // waves per block
   constexpr int kMaxRuns = kMaxW * 4;                  // 64-omega runs per block (M <= 4)
 
@@ -804,7 +815,7 @@ k_spectrum(const KParams p) {
         int step_on = 1;
         int st_wide = 0;
         Amp st_amp[NA];
-        double st_tau = 0.0, st_thr = 0.0;
+        double st_tau = 0.0, st_thr = 0.0, st_prev = 0.0;
         float st_rot[2] = {1.f, 0.f};
         if (tid < cnt) {
           // ---- stage: omega-independent part of step s = t0 + tid, in FP64 ----
@@ -815,13 +826,16 @@ k_spectrum(const KParams p) {
           const Geom q = geom();
           const double gx = q.gx, gy = q.gy, gz = q.gz;
           const double thx = q.thx, thy = q.thy, thz = q.thz, phx = q.phx, phy = q.phy;
-          double tau, tau_prev = 0.0;
+          double tau;
+          // tau_{s-1} (phase_{-1} = 0 at the first step): recomputed here by the
+          // tile's first thread, or by every thread without the exchange
+          const bool own_prev = kSplit && s > 0 && (!kExch || tid == 0);
           if (MODE == SR_MODE_FAR) {
             // kernel_farfield.cl:65-94
             tau = time - (xs * gx + ys * gy + zs * gz);
-            if (kSplit && s > 0) {
+            if (own_prev) {
               const double tp = (double)(uint32_t)(pt.its + (uint32_t)(s - 1)) * dt;
-              tau_prev = tp - (p.x[gs - 1] * gx + p.y[gs - 1] * gy + p.z[gs - 1] * gz);
+              st_prev = tp - (p.x[gs - 1] * gx + p.y[gs - 1] * gy + p.z[gs - 1] * gz);
             }
             double ax, ay, az, mx, my, mz;
             if (p.pre) {
@@ -862,10 +876,10 @@ k_spectrum(const KParams p) {
             const double rx = gx - xs, ry = gy - ys, rz = gz - zs;
             const double R = sqrt(rx * rx + ry * ry + rz * rz);
             tau = time + R;
-            if (kSplit && s > 0) {
+            if (own_prev) {
               const double tp = (double)(uint32_t)(pt.its + (uint32_t)(s - 1)) * dt;
               const double qx = gx - p.x[gs - 1], qy = gy - p.y[gs - 1], qz = gz - p.z[gs - 1];
-              tau_prev = tp + sqrt(qx * qx + qy * qy + qz * qz);
+              st_prev = tp + sqrt(qx * qx + qy * qy + qz * qz);
             }
             const double rInv = 1.0 / R;
             const double nx = rInv * rx, ny = rInv * ry, nz = rInv * rz;
@@ -879,7 +893,17 @@ k_spectrum(const KParams p) {
           }
           st_tau = tau;
           if (kTab) st_wide = fabs(tau) * om_abs_max >= 0.5 * kTabMaxPhase;
-          if (kSplit) {
+          if (kRec) sincos_rev(rot_rev * tau, &st_rot[1], &st_rot[0]);
+          if (kExch) s_tau[tid] = tau;   // in step order, for the next thread
+        }
+        if constexpr (kSplit) {
+          if (kExch) {
+            // every tau of the tile is in LDS; the staged copies are written
+            // after the step sort's barriers (or, unsorted, over the same value)
+            __syncthreads();
+            if (tid > 0 && tid < cnt) st_prev = s_tau[tid - 1];
+          }
+          if (tid < cnt) {
             // |omega tau_s - omega tau_{s-1}| < pi  <=>  |omega|/2pi < (1/2)/|tau_s - tau_{s-1}|
             // (phase_{-1} = 0 at the first step); 1/0 gives +inf: always active
             // (split: the hardware reciprocal with one Newton step, as for c2;
@@ -889,16 +913,15 @@ k_spectrum(const KParams p) {
             // step alone would make it fma(-0, inf, 1) = NaN)
             double thr;
             if (SR_SPLIT_RCP) {
-              const double d = fabs(tau - tau_prev);
+              const double d = fabs(st_tau - st_prev);
               const double r = __builtin_amdgcn_rcp(d);
               thr = (d == 0.0) ? INFINITY : 0.5 * fma(r, fma(-d, r, 1.0), r);
             } else {
-              thr = 0.5 / fabs(tau - tau_prev);
+              thr = 0.5 / fabs(st_tau - st_prev);
             }
             st_thr = thr;
             step_on = thr > w_hi;
           }
-          if (kRec) sincos_rev(rot_rev * tau, &st_rot[1], &st_rot[0]);
         }
         // compaction (tiles without a snapshot record): bucket_s = number of runs
         // that keep some omega at step s; a stable counting sort puts the steps
