@@ -332,6 +332,7 @@ k_spectrum(const KParams p) {
   constexpr bool kTab = !kSplit && (SR_F64_TABLE != 0);
   __shared__ double2 s_sct[kTab ? kSinCosTab : 1];
   __shared__ double s_wmax[kTab ? kMaxW : 1];
+  __shared__ int s_vote[kMaxW];   // per-wave votes of a block-wide any/all
 
   const int T = blockDim.x;
   const int tid = threadIdx.x;
@@ -977,14 +978,26 @@ k_spectrum(const KParams p) {
 #pragma unroll
           for (int c = 0; c < NA; c++) s_amp[c][pos] = st_amp[c];
         }
-        // rec: the tile needs no masking when every step keeps the whole block
+        // rec: the tile needs no masking when every step keeps the whole block.
+        // The block votes through one ballot per wave and an LDS word per wave
+        // (the block is one-dimensional: unlike the library's work-group
+        // reductions this keeps no y/z work-item ids alive, which cost the FP64
+        // near-field variant two spilled registers)
         bool tile_on = false;
-        if constexpr (kRec)
-          tile_on = __syncthreads_and(step_on) != 0;
-        else if constexpr (kTab)
-          tile_wide = uni(__syncthreads_or(st_wide)) != 0;
-        else
+        if constexpr (kRec || kTab) {
+          const bool pred = kRec ? (step_on == 0) : (st_wide != 0);   // rec: any step not keeping all
+          const uint64_t b = __ballot(pred);
+          if (lane == 0) s_vote[wv] = (b != 0) ? 1 : 0;
           __syncthreads();
+          int any = 0;
+          for (int w = 0; w < W; w++) any |= s_vote[w];
+          if constexpr (kRec)
+            tile_on = uni(any) == 0;
+          else
+            tile_wide = uni(any) != 0;
+        } else {
+          __syncthreads();
+        }
 
         // ---- sweep, split at snapshot records falling inside the tile ----
         if (sort_tile) {
