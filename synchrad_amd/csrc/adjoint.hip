@@ -73,7 +73,8 @@ struct AParams {
   const double *dS[3];
   // fields of particles [chunk_base, chunk_base + chunk_n): F[6][chunk_n][n_grid]
   const double *F;
-  // step data [a_x, a_y, a_z, bbar_x, bbar_y, bbar_z][n_points] (spectrum.hip step_prep)
+  // step data, one record per sample [n_points][a_x, a_y, a_z, bbar_x, bbar_y, bbar_z]
+  // (spectrum.hip step_prep)
   const double *pre;
   int64_t chunk_base, chunk_n, n_grid;
   // decomposition and workspace
@@ -232,9 +233,9 @@ __global__ void __launch_bounds__(kMaxT) k_adjoint(const AParams p) {
         xp = p.x[gs - 1]; yp = p.y[gs - 1]; zp = p.z[gs - 1];
       }
       // direction-independent part of the step (kernel_farfield.cl:74-83)
-      const int64_t np = p.n_points;
-      const double a[3] = {p.pre[gs], p.pre[np + gs], p.pre[2 * np + gs]};
-      const double m[3] = {p.pre[3 * np + gs], p.pre[4 * np + gs], p.pre[5 * np + gs]};
+      const double *rec = p.pre + 6 * gs;
+      const double a[3] = {rec[0], rec[1], rec[2]};
+      const double m[3] = {rec[3], rec[4], rec[5]};
       double acc9[kSlabWords];
 #pragma unroll
       for (int k = 0; k < kSlabWords; k++) acc9[k] = 0.0;

@@ -129,8 +129,10 @@ struct KParams {
   // (ip - part_base) * part_stride past the usual index; 0 otherwise
   int64_t part_base, part_stride;
   // direction-independent step data from k_step_prep, or nullptr (the stage
-  // then computes it): far [a_x, a_y, a_z, bbar_x, bbar_y, bbar_z][n_points],
-  // near [b_x, b_y, b_z][n_points]
+  // then computes it), one record per sample: far [n_points][a_x, a_y, a_z,
+  // bbar_x, bbar_y, bbar_z] (48 B, 16-byte aligned), near [n_points][b_x, b_y,
+  // b_z].  One base address and fixed offsets: the stage reads a record with
+  // three 16-byte loads and keeps no per-array base registers
   const double *pre;
   int64_t n_points;
   // tiling
@@ -840,9 +842,10 @@ k_spectrum(const KParams p) {
             }
             double ax, ay, az, mx, my, mz;
             if (p.pre) {
-              const int64_t np = p.n_points;
-              ax = p.pre[gs]; ay = p.pre[np + gs]; az = p.pre[2 * np + gs];
-              mx = p.pre[3 * np + gs]; my = p.pre[4 * np + gs]; mz = p.pre[5 * np + gs];
+              const double2 *r = (const double2 *)(p.pre + 6 * gs);
+              const double2 r0 = r[0], r1 = r[1], r2 = r[2];
+              ax = r0.x; ay = r0.y; az = r1.x;
+              mx = r1.y; my = r2.x; mz = r2.y;
             } else {
               step_am(p.ux, p.uy, p.uz, gs, dtInv, &ax, &ay, &az, &mx, &my, &mz);
             }
@@ -868,8 +871,8 @@ k_spectrum(const KParams p) {
             // kernel_nearfield.cl:67-88; omega of c1 = omega v1 is applied in the sums
             double b0x, b0y, b0z;
             if (p.pre) {
-              const int64_t np = p.n_points;
-              b0x = p.pre[gs]; b0y = p.pre[np + gs]; b0z = p.pre[2 * np + gs];
+              const double *r = p.pre + 3 * gs;
+              b0x = r[0]; b0y = r[1]; b0z = r[2];
             } else {
               const double g0 = gamma_inv(p.ux[gs], p.uy[gs], p.uz[gs]);
               b0x = p.ux[gs] * g0; b0y = p.uy[gs] * g0; b0z = p.uz[gs] * g0;
@@ -1161,11 +1164,14 @@ __global__ void k_step_prep(const double *ux, const double *uy, const double *uz
   if (MODE == SR_MODE_FAR) {
     double ax = 0.0, ay = 0.0, az = 0.0, mx = 0.0, my = 0.0, mz = 0.0;
     if (i + 1 < n) step_am(ux, uy, uz, i, 1.0 / dt, &ax, &ay, &az, &mx, &my, &mz);
-    pre[i] = ax; pre[n + i] = ay; pre[2 * n + i] = az;
-    pre[3 * n + i] = mx; pre[4 * n + i] = my; pre[5 * n + i] = mz;
+    double2 *r = (double2 *)(pre + 6 * i);
+    r[0] = make_double2(ax, ay);
+    r[1] = make_double2(az, mx);
+    r[2] = make_double2(my, mz);
   } else {
     const double g0 = gamma_inv(ux[i], uy[i], uz[i]);
-    pre[i] = ux[i] * g0; pre[n + i] = uy[i] * g0; pre[2 * n + i] = uz[i] * g0;
+    double *r = pre + 3 * i;
+    r[0] = ux[i] * g0; r[1] = uy[i] * g0; r[2] = uz[i] * g0;
   }
 }
 
@@ -1455,7 +1461,9 @@ void slice_particles(KParams &kp, int slices, int64_t n_particles, const int64_t
 // workspace of a batch call: the slices' partial spectra, then the
 // direction-independent step data
 size_t scratch_bytes(const Plan &pl, int n_out, int64_t n_grid, uint32_t n_snaps) {
-  return pl.slices > 1 ? (size_t)pl.slices * n_out * (size_t)n_grid * n_snaps * sizeof(double) : 0;
+  // rounded to 256 bytes: the step records after it are read with 16-byte loads
+  return pl.slices > 1 ? ((size_t)pl.slices * n_out * (size_t)n_grid * n_snaps * sizeof(double) + 255) / 256 * 256
+                       : 0;
 }
 size_t pre_bytes(int mode, int64_t n_points) {
   return (size_t)(mode == SR_MODE_FAR ? 6 : 3) * (size_t)n_points * sizeof(double);
