@@ -481,14 +481,12 @@ k_spectrum(const KParams p) {
       }
       // published by the barrier that opens the first step tile
     }
-    double *dst[kMaxOut];
-#pragma unroll
-    for (int c = 0; c < kMaxOut; c++)
-      dst[c] = (c < p.n_out)
-                   ? ((p.slices > 1 && p.part_stride == 0)
-                        ? p.scratch + ((int64_t)q * p.n_out + c) * p.n_grid * p.n_snaps
-                        : p.out[c])
-                   : nullptr;
+    // output c of this tile (slices: the slice's partial spectra), formed where it is written
+    auto dst = [&](int c) -> double * {
+      return (p.slices > 1 && p.part_stride == 0)
+                 ? p.scratch + ((int64_t)q * p.n_out + c) * p.n_grid * p.n_snaps
+                 : p.out[c];
+    };
 
     if (p.acc_lds) {
 #pragma unroll
@@ -589,7 +587,7 @@ k_spectrum(const KParams p) {
                             (ip - p.part_base) * p.part_stride;
 #pragma unroll
           for (int c = 0; c < kMaxOut; c++)
-            if (c < p.n_out) dst[c][g] += v[c];
+            if (c < p.n_out) dst(c)[g] += v[c];
         }
       };
       auto emit = [&](uint32_t slot) {
@@ -1059,7 +1057,7 @@ k_spectrum(const KParams p) {
 #pragma unroll 1
       for (int k = 0; k < M; k++) {
         const int wk = omega_slot(k);
-        if (wk < (int)p.n_omega) dst[0][(int64_t)dir * p.n_omega + wk] += s_acc[k * T];
+        if (wk < (int)p.n_omega) dst(0)[(int64_t)dir * p.n_omega + wk] += s_acc[k * T];
       }
     }
   }
