@@ -299,7 +299,9 @@ struct WavesPerEU {
                              : (M >= 4 ? 2 : SR_DIRECT_WAVES);
 };
 
-template <int MODE, int PREC, int SWEEP, int M>
+// PRE: the step data come from the pre-pass records (batch calls); without
+// them (the per-particle and no-allocation calls) the stage forms them from u
+template <int MODE, int PREC, int SWEEP, int M, bool PRE>
 __global__ void __launch_bounds__(kMaxT)
 __attribute__((amdgpu_waves_per_eu(WavesPerEU<MODE, PREC, SWEEP, M>::value)))
 k_spectrum(const KParams p) {
@@ -839,7 +841,7 @@ k_spectrum(const KParams p) {
               st_prev = tp - (p.x[gs - 1] * gx + p.y[gs - 1] * gy + p.z[gs - 1] * gz);
             }
             double ax, ay, az, mx, my, mz;
-            if (p.pre) {
+            if constexpr (PRE) {
               const double2 *r = (const double2 *)(p.pre + 6 * gs);
               const double2 r0 = r[0], r1 = r[1], r2 = r[2];
               ax = r0.x; ay = r0.y; az = r1.x;
@@ -868,7 +870,7 @@ k_spectrum(const KParams p) {
           } else {
             // kernel_nearfield.cl:67-88; omega of c1 = omega v1 is applied in the sums
             double b0x, b0y, b0z;
-            if (p.pre) {
+            if constexpr (PRE) {
               const double *r = p.pre + 3 * gs;
               b0x = r[0]; b0y = r[1]; b0z = r[2];
             } else {
@@ -1350,8 +1352,12 @@ hipError_t launch_t(const KParams &kp, int64_t grid, int threads, hipStream_t st
   const size_t lds = stage_lds_bytes<MODE, PREC, SWEEP>(threads) +
                      ((PREC == SR_PREC_SPLIT) ? kahan_lds_bytes<MODE, M>(threads) : 0) +
                      (kp.acc_lds ? acc_lds_bytes<M>(threads) : 0);
-  hipLaunchKernelGGL((k_spectrum<MODE, PREC, SWEEP, M>), dim3((unsigned)grid), dim3(threads), lds, st,
-                     kp);
+  if (kp.pre)
+    hipLaunchKernelGGL((k_spectrum<MODE, PREC, SWEEP, M, true>), dim3((unsigned)grid), dim3(threads), lds, st,
+                       kp);
+  else
+    hipLaunchKernelGGL((k_spectrum<MODE, PREC, SWEEP, M, false>), dim3((unsigned)grid), dim3(threads), lds, st,
+                       kp);
   return hipGetLastError();
 }
 
