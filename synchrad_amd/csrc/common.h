@@ -67,6 +67,14 @@ __device__ __forceinline__ V late(V v) {
   asm volatile("" : "+v"(v));
   return v;
 }
+// The same for a wave-uniform value, kept in a scalar register: a flag read
+// here is tested here by a scalar branch, not carried through the loops as a
+// per-lane copy.
+template <typename V>
+__device__ __forceinline__ V late_s(V v) {
+  asm volatile("" : "+s"(v));
+  return v;
+}
 
 // Division of n < 2^31 by a divisor fixed for a launch, as a multiply and a
 // shift (Granlund and Montgomery, "Division by invariant integers using

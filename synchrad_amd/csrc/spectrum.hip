@@ -490,7 +490,7 @@ k_spectrum(const KParams p) {
                  : p.out[c];
     };
 
-    if (p.acc_lds) {
+    if (late_s(p.acc_lds)) {
 #pragma unroll
       for (int k = 0; k < M; k++) s_acc[k * T] = 0.0;
     }
@@ -582,7 +582,7 @@ k_spectrum(const KParams p) {
             }
           }
         }
-        if (p.acc_lds) {
+        if (late_s(p.acc_lds)) {
           s_acc[k * T] += v[0];
         } else {
           const int64_t g = (int64_t)slot * p.n_grid + (int64_t)dir * p.n_omega + late(wk) +
@@ -972,6 +972,7 @@ k_spectrum(const KParams p) {
           pos = (bucket > 0) ? s_off[(R - bucket) * W + wv] + __popcll(mine & below) : -1;
         }
         if (tid < cnt && pos >= 0) {
+          pos = late(pos);   // the staged slots' LDS addresses formed here
           s_tau[pos] = st_tau;
           if (kSplit) s_thr[pos] = st_thr;
           if (kRec) {
@@ -1055,7 +1056,7 @@ k_spectrum(const KParams p) {
         cur.advance(pt, p.n_snaps, last);
       }
     }
-    if (p.acc_lds) {
+    if (late_s(p.acc_lds)) {
 #pragma unroll 1
       for (int k = 0; k < M; k++) {
         const int wk = omega_slot(k);
