@@ -183,6 +183,30 @@ __device__ __forceinline__ void accumulate(X *acc, const X *a, X cn, X sn, X w) 
     }
   }
 }
+// The split sweeps' register partials of one frequency.  Far: as accumulate.
+// Near: the two amplitude parts kept apart, [v2 cos | v2 sin | v1 sin | v1 cos]
+// (12 sums, 12 FMAs per step and no multiplication by omega); the Kahan fold
+// forms the field's [Re F | Im F] = [v2 cos - omega v1 sin | v2 sin + omega
+// v1 cos] from them once per tile.
+template <int MODE, int NA>
+__device__ __forceinline__ void split_acc(float *acc, const float *a, float cn, float sn) {
+  if constexpr (MODE == SR_MODE_FAR) {
+#pragma unroll
+    for (int c = 0; c < NA; c++) {
+      acc[c] = fmx(a[c], cn, acc[c]);
+      acc[NA + c] = fmx(a[c], sn, acc[NA + c]);
+    }
+  } else {
+#pragma unroll
+    for (int c = 0; c < 3; c++) {
+      acc[c] = fmx(a[3 + c], cn, acc[c]);
+      acc[3 + c] = fmx(a[3 + c], sn, acc[3 + c]);
+      acc[6 + c] = fmx(a[c], sn, acc[6 + c]);
+      acc[9 + c] = fmx(a[c], cn, acc[9 + c]);
+    }
+  }
+}
+
 // Dynamic LDS after the Kahan state: the per-thread particle sums of the
 // single output when KParams::acc_lds is set, [M][T] doubles.
 template <int M>
@@ -316,6 +340,7 @@ k_spectrum(const KParams p) {
   // the two agree to rounding).  Near: v1 = (b - n)/R and v2 = n/R^2.
   constexpr int NA = amp_words<MODE, PREC>();
   constexpr int NC = (MODE == SR_MODE_FAR) ? 2 * NA : 6;   // real accumulators per omega
+  constexpr int NP = (MODE == SR_MODE_FAR) ? NC : 12;      // split register partials per omega
   constexpr bool kCompact = kSplit && kRec && (SR_COMPACT != 0);
   // (the direct sweep has no step sort whose barriers it could share: it
   // recomputes tau_{s-1}, which measured 0.5 % faster there)
@@ -509,15 +534,14 @@ k_spectrum(const KParams p) {
       // sum and compensation in LDS
       double prev[M];
       double fs[M][NC];
-      float ps[M][NC];
+      float ps[M][NP];
 #pragma unroll
       for (int k = 0; k < M; k++) {
         prev[k] = 0.0;
 #pragma unroll
-        for (int c = 0; c < NC; c++) {
-          fs[k][c] = 0.0;
-          ps[k][c] = 0.f;
-        }
+        for (int c = 0; c < NC; c++) fs[k][c] = 0.0;
+#pragma unroll
+        for (int c = 0; c < NP; c++) ps[k][c] = 0.f;
       }
       if (kSplit) {
 #pragma unroll
@@ -631,7 +655,7 @@ k_spectrum(const KParams p) {
             cm = act ? cn : 0.f;
             sm = act ? sn : 0.f;
           }
-          accumulate<MODE, NA>(ps[k], a, cm, sm, omf[k]);
+          split_acc<MODE, NA>(ps[k], a, cm, sm);
           if (k + 1 < K) {
             const float cn2 = __builtin_fmaf(cn, er, -sn * ei);
             sn = __builtin_fmaf(cn, ei, sn * er);
@@ -740,7 +764,7 @@ k_spectrum(const KParams p) {
               float af[NA];
 #pragma unroll
               for (int c = 0; c < NA; c++) af[c] = (float)a[c];
-              accumulate<MODE, NA>(ps[k], af, cn, sn, omf[k]);
+              split_acc<MODE, NA>(ps[k], af, cn, sn);
             } else {
               double sn, cn;
               if (kTab)
@@ -782,7 +806,7 @@ k_spectrum(const KParams p) {
                 if (valid[k] && fabs(om_rev[k]) < hq[q]) {
                   float sn, cn;
                   sincos_rev(om_rev[k] * tq[q], &sn, &cn);
-                  accumulate<MODE, NA>(ps[k], a, cn, sn, omf[k]);
+                  split_acc<MODE, NA>(ps[k], a, cn, sn);
                 }
               }
             }
@@ -799,17 +823,29 @@ k_spectrum(const KParams p) {
       auto kahan_fold = [&]() {
         if (!kSplit) return;
 #pragma unroll
-        for (int k = 0; k < M; k++)
+        for (int k = 0; k < M; k++) {
+          float part[NC];
+#pragma unroll
+          for (int c = 0; c < NC; c++) part[c] = ps[k][c];
+          if constexpr (MODE != SR_MODE_FAR) {
+#pragma unroll
+            for (int c = 0; c < 3; c++) {
+              part[c] = fmx(-omf[k], ps[k][6 + c], part[c]);
+              part[3 + c] = fmx(omf[k], ps[k][9 + c], part[3 + c]);
+            }
+          }
 #pragma unroll
           for (int c = 0; c < NC; c++) {
             float &S = s_ks[(c * M + k) * T];
             float &C = s_kc[(c * M + k) * T];
-            const float yv = ps[k][c] - C;
+            const float yv = part[c] - C;
             const float tv = S + yv;
             C = (tv - S) - yv;
             S = tv;
-            ps[k][c] = 0.f;
           }
+#pragma unroll
+          for (int c = 0; c < NP; c++) ps[k][c] = 0.f;
+        }
       };
 
       for (int64_t t0 = 0; t0 < n_act; t0 += T) {
