@@ -338,7 +338,8 @@ def test_autotuned_plan_matches_oracle():
     ax = wl.axes
     grid = engine.DeviceGrid("far", ax["omega"], ax["theta"], ax["phi"])
     plan, rows = autotune.tune(batch, grid, comp="total", precision="float", target_evals=1e8)
-    assert len(rows) == len(autotune.candidates("far", "float", True, 128)) and all(r["ms"] > 0 for r in rows)
+    first = [r for r in rows if r["stage"] == "sample"]
+    assert len(first) == len(autotune.candidates("far", "float", True, 128)) and all(r["ms"] > 0 for r in rows)
     out = engine.spectrum(batch, grid, comp="total", precision="float", tuning=plan)[0]
     got = out.cpu().numpy().swapaxes(-1, -3)
     assert rel_l2(got, ref["total"]) < SPLIT_TOL
