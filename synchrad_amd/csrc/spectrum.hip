@@ -1329,9 +1329,9 @@ Plan make_plan(const sr_grid &g, const sr_opts &o, int64_t n_particles, uint32_t
       const double eff = (double)n_om / (double)(chunks * run);
       if (eff < 0.75) continue;
       // far-field recurrence: 4 omegas per thread in 2-wave blocks (one
-      // 512-omega chunk per direction, 161 registers, 3 waves per SIMD:
-      // 5934 G evals/s on the betatron beam against 5267 for 2 omegas x 2
-      // waves and 4871 for 2 x 4); elsewhere M = 2 measured fastest.  Blocks
+      // 512-omega chunk per direction, 3 waves per SIMD; when chosen: 5934 G
+      // evals/s on the betatron beam against 5267 for 2 omegas x 2 waves and
+      // 4871 for 2 x 4); elsewhere M = 2 measured fastest.  Blocks
       // of 2 waves: with the tiles launched heaviest first, the finer tiles
       // balance better (+8 % over 4 waves)
       const bool rec_far = pl.sweep == SWEEP_REC && g.mode == SR_MODE_FAR;
