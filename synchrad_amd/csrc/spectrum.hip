@@ -294,6 +294,11 @@ struct LdsRows {
 // so a wave sweeps only those (a stable counting sort per tile).  The direct
 // sweep measured slower with it (the sort's registers cost it a wave per SIMD
 // and skipped steps are already cheap there), so it keeps the tile order.
+// recurrence sweep: a thread's third and fourth omegas by the three-term
+// recurrence of equally spaced angles instead of a complex multiplication
+#ifndef SR_REC_CHEB
+#define SR_REC_CHEB 1
+#endif
 #ifndef SR_COMPACT
 #define SR_COMPACT 1
 #endif
@@ -643,10 +648,17 @@ k_spectrum(const KParams p) {
       // masked otherwise; 2: only the K-th omega is masked (sorted tiles, where
       // the runs below the step's last kept run keep all their omegas).
       // rec_acc: the accumulations of one step from its base sin/cos
+      // The second omega comes from the first by one complex multiplication;
+      // the third and fourth by the three-term recurrence z_{k+1} = 2 cos(d)
+      // z_k - z_{k-1} of equally spaced angles (two FMAs instead of two
+      // multiplications and two FMAs; over two terms its error growth is
+      // that of the multiplication's within a factor 3)
       auto rec_acc = [&](auto Kc, auto Mc, float sn, float cn, double thr, float er, float ei,
                          const float *a) {
         constexpr int K = decltype(Kc)::value;
         constexpr int MASK = decltype(Mc)::value;
+        float cp = 0.f, sp = 0.f;   // z_{k-1}
+        const float c2 = (SR_REC_CHEB && K >= 3) ? er + er : 0.f;
 #pragma unroll
         for (int k = 0; k < K; k++) {
           float cm = cn, sm = sn;
@@ -657,9 +669,18 @@ k_spectrum(const KParams p) {
           }
           split_acc<MODE, NA>(ps[k], a, cm, sm);
           if (k + 1 < K) {
-            const float cn2 = __builtin_fmaf(cn, er, -sn * ei);
-            sn = __builtin_fmaf(cn, ei, sn * er);
+            float cn2, sn2;
+            if (SR_REC_CHEB && k >= 1) {
+              cn2 = __builtin_fmaf(c2, cn, -cp);
+              sn2 = __builtin_fmaf(c2, sn, -sp);
+            } else {
+              cn2 = __builtin_fmaf(cn, er, -sn * ei);
+              sn2 = __builtin_fmaf(cn, ei, sn * er);
+            }
+            cp = cn;
+            sp = sn;
             cn = cn2;
+            sn = sn2;
           }
         }
       };
