@@ -232,6 +232,18 @@ constexpr size_t stage_lds_bytes(int T) {
                       amp_words<MODE, PREC>() * (split ? sizeof(float) : sizeof(double)));
 }
 
+// one staged step of the recurrence sweep (split): tau, threshold, the
+// rotation e^{i T domega tau} and the amplitude words in one 16-byte aligned
+// record (32 B far, 48 B near), so that the sweep reads a group of steps from
+// one base address with immediate offsets (the same bytes as the separate
+// arrays of the other sweeps)
+template <int NA>
+struct alignas(16) RecStep {
+  double tau, thr;
+  float er, ei;
+  float amp[NA];
+};
+
 // rows of a [n][T] LDS array: rows[c][j]
 template <typename X>
 struct LdsRows {
@@ -254,9 +266,6 @@ struct LdsRows {
 #endif
 #ifndef SR_REC_WAVES_FAR4
 #define SR_REC_WAVES_FAR4 2
-#endif
-#ifndef SR_REC_GROUP
-#define SR_REC_GROUP 4
 #endif
 // recurrence groups of 4 steps: all base sin/cos first, then the sums
 #ifndef SR_REC_PIPE
@@ -389,12 +398,14 @@ k_spectrum(const KParams p) {
     __syncthreads();
     for (int w = 0; w < W; w++) om_abs_max = fmax(om_abs_max, s_wmax[w]);
   }
-  // dynamic LDS: the staged tile (s_tau[T], s_thr[T], s_rot[2][T],
-  // s_amp[NA][T]), then the Kahan state, then the particle sums
+  // dynamic LDS: the staged tile (recurrence: s_rec[T]; else s_tau[T],
+  // s_thr[T] (split), s_amp[NA][T]), then the Kahan state, then the particle sums
+  static_assert(!kRec || sizeof(RecStep<NA>) == 2 * sizeof(double) + (2 + NA) * sizeof(float),
+                "a recurrence record holds exactly the staged step's bytes");
+  RecStep<NA> *const s_rec = (RecStep<NA> *)s_dyn_raw;
   double *const s_tau = (double *)s_dyn_raw;
   double *const s_thr = s_tau + T;
-  const LdsRows<float> s_rot{(float *)(s_tau + (kSplit ? 2 : 1) * T), T};
-  const LdsRows<Amp> s_amp{(Amp *)((char *)s_rot.base + (kRec ? 2 * T * sizeof(float) : 0)), T};
+  const LdsRows<Amp> s_amp{(Amp *)(s_tau + (kSplit ? 2 : 1) * T), T};
   float *const s_dyn = (float *)(s_dyn_raw + stage_lds_bytes<MODE, PREC, SWEEP>(T));
   float *const s_ks = s_dyn + tid;                       // S[c*M+k] at s_ks[(c*M+k)*T]
   float *const s_kc = s_dyn + (size_t)NC * M * T + tid;  // C likewise
@@ -690,67 +701,51 @@ k_spectrum(const KParams p) {
         sincos_rev(om0_rev * tau, &sn, &cn);
         rec_acc(Kc, Mc, sn, cn, thr, er, ei, a);
       };
-      // steps [j, jend) of the staged tile, single steps up to a multiple of 4,
-      // then groups of 4 fetched with 16-byte LDS reads, then the tail
-      auto rec_sweep = [&](auto Kc, auto Mc, int &j, int jend) {
-        const int jg = (SR_REC_GROUP == 4) ? min(jend, (j + 3) & ~3) : jend;
-        for (; j < jg; j++) {
-          float a[NA];
-#pragma unroll
-          for (int c = 0; c < NA; c++) a[c] = (float)s_amp[c][j];
-          rec_step(Kc, Mc, s_tau[j], s_thr[j], s_rot[0][j], s_rot[1][j], a);
+      // staged step jj of the recurrence sweep: two (far) or three (near)
+      // 16-byte reads of its record
+      auto rec_load = [&](int jj, double &tau, double &thr, float &er, float &ei, float *a) {
+        const RecStep<NA> &r = s_rec[jj];
+        const double2 tt = *(const double2 *)&r.tau;
+        const float4 u = *(const float4 *)&r.er;
+        tau = tt.x;
+        thr = tt.y;
+        er = u.x;
+        ei = u.y;
+        a[0] = u.z;
+        a[1] = u.w;
+        if constexpr (NA > 2) {
+          const float4 w = *(const float4 *)&r.amp[2];
+          a[2] = w.x;
+          a[3] = w.y;
+          a[4] = w.z;
+          a[5] = w.w;
         }
+      };
+      // steps [j, jend) of the staged tile: groups of 4, then the tail
+      auto rec_sweep = [&](auto Kc, auto Mc, int &j, int jend) {
         for (; j + 4 <= jend; j += 4) {
-          const double2 t01 = *(const double2 *)&s_tau[j];
-          const double2 t23 = *(const double2 *)&s_tau[j + 2];
-          const double2 h01 = *(const double2 *)&s_thr[j];
-          const double2 h23 = *(const double2 *)&s_thr[j + 2];
-          const float4 er = *(const float4 *)&s_rot[0][j];
-          const float4 ei = *(const float4 *)&s_rot[1][j];
-          float4 av[NA];
+          double tq[4], hq[4];
+          float erq[4], eiq[4], aq[4][NA];
 #pragma unroll
-          for (int c = 0; c < NA; c++) av[c] = *(const float4 *)&s_amp[c][j];
-          float a[NA];
+          for (int q = 0; q < 4; q++) rec_load(j + q, tq[q], hq[q], erq[q], eiq[q], aq[q]);
           if (SR_REC_PIPE && decltype(Mc)::value != 1) {
             // the four base phases and sin/cos first (independent chains), then
             // the accumulations
             float sn[4], cn[4];
-            sincos_rev(om0_rev * t01.x, &sn[0], &cn[0]);
-            sincos_rev(om0_rev * t01.y, &sn[1], &cn[1]);
-            sincos_rev(om0_rev * t23.x, &sn[2], &cn[2]);
-            sincos_rev(om0_rev * t23.y, &sn[3], &cn[3]);
 #pragma unroll
-            for (int c = 0; c < NA; c++) a[c] = av[c].x;
-            rec_acc(Kc, Mc, sn[0], cn[0], h01.x, er.x, ei.x, a);
+            for (int q = 0; q < 4; q++) sincos_rev(om0_rev * tq[q], &sn[q], &cn[q]);
 #pragma unroll
-            for (int c = 0; c < NA; c++) a[c] = av[c].y;
-            rec_acc(Kc, Mc, sn[1], cn[1], h01.y, er.y, ei.y, a);
-#pragma unroll
-            for (int c = 0; c < NA; c++) a[c] = av[c].z;
-            rec_acc(Kc, Mc, sn[2], cn[2], h23.x, er.z, ei.z, a);
-#pragma unroll
-            for (int c = 0; c < NA; c++) a[c] = av[c].w;
-            rec_acc(Kc, Mc, sn[3], cn[3], h23.y, er.w, ei.w, a);
+            for (int q = 0; q < 4; q++) rec_acc(Kc, Mc, sn[q], cn[q], hq[q], erq[q], eiq[q], aq[q]);
           } else {
 #pragma unroll
-            for (int c = 0; c < NA; c++) a[c] = av[c].x;
-            rec_step(Kc, Mc, t01.x, h01.x, er.x, ei.x, a);
-#pragma unroll
-            for (int c = 0; c < NA; c++) a[c] = av[c].y;
-            rec_step(Kc, Mc, t01.y, h01.y, er.y, ei.y, a);
-#pragma unroll
-            for (int c = 0; c < NA; c++) a[c] = av[c].z;
-            rec_step(Kc, Mc, t23.x, h23.x, er.z, ei.z, a);
-#pragma unroll
-            for (int c = 0; c < NA; c++) a[c] = av[c].w;
-            rec_step(Kc, Mc, t23.y, h23.y, er.w, ei.w, a);
+            for (int q = 0; q < 4; q++) rec_step(Kc, Mc, tq[q], hq[q], erq[q], eiq[q], aq[q]);
           }
         }
         for (; j < jend; j++) {
-          float a[NA];
-#pragma unroll
-          for (int c = 0; c < NA; c++) a[c] = (float)s_amp[c][j];
-          rec_step(Kc, Mc, s_tau[j], s_thr[j], s_rot[0][j], s_rot[1][j], a);
+          double tau, thr;
+          float er, ei, a[NA];
+          rec_load(j, tau, thr, er, ei, a);
+          rec_step(Kc, Mc, tau, thr, er, ei, a);
         }
       };
 
@@ -955,14 +950,14 @@ k_spectrum(const KParams p) {
           st_tau = tau;
           if (kTab) st_wide = fabs(tau) * om_abs_max >= 0.5 * kTabMaxPhase;
           if (kRec) sincos_rev(rot_rev * tau, &st_rot[1], &st_rot[0]);
-          if (kExch) s_tau[tid] = tau;   // in step order, for the next thread
+          if constexpr (kExch) s_rec[tid].tau = tau;   // in step order, for the next thread
         }
         if constexpr (kSplit) {
           if (kExch) {
             // every tau of the tile is in LDS; the staged copies are written
             // after the step sort's barriers (or, unsorted, over the same value)
             __syncthreads();
-            if (tid > 0 && tid < cnt) st_prev = s_tau[tid - 1];
+            if (tid > 0 && tid < cnt) st_prev = s_rec[tid - 1].tau;
           }
           if (tid < cnt) {
             // |omega tau_s - omega tau_{s-1}| < pi  <=>  |omega|/2pi < (1/2)/|tau_s - tau_{s-1}|
@@ -1030,14 +1025,18 @@ k_spectrum(const KParams p) {
         }
         if (tid < cnt && pos >= 0) {
           pos = late(pos);   // the staged slots' LDS addresses formed here
-          s_tau[pos] = st_tau;
-          if (kSplit) s_thr[pos] = st_thr;
-          if (kRec) {
-            s_rot[0][pos] = st_rot[0];
-            s_rot[1][pos] = st_rot[1];
-          }
+          if constexpr (kRec) {
+            RecStep<NA> &r = s_rec[pos];
+            *(double2 *)&r.tau = make_double2(st_tau, st_thr);
+            *(float4 *)&r.er = make_float4(st_rot[0], st_rot[1], (float)st_amp[0], (float)st_amp[1]);
+            if constexpr (NA > 2)
+              *(float4 *)&r.amp[2] = make_float4((float)st_amp[2], (float)st_amp[3], (float)st_amp[4], (float)st_amp[5]);
+          } else {
+            s_tau[pos] = st_tau;
+            if (kSplit) s_thr[pos] = st_thr;
 #pragma unroll
-          for (int c = 0; c < NA; c++) s_amp[c][pos] = st_amp[c];
+            for (int c = 0; c < NA; c++) s_amp[c][pos] = st_amp[c];
+          }
         }
         // rec: the tile needs no masking when every step keeps the whole block.
         // The block votes through one ballot per wave and an LDS word per wave
