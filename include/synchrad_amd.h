@@ -126,8 +126,9 @@ typedef struct sr_grid {
 typedef struct sr_opts {
   int32_t comp;             /* SR_COMP_* */
   int32_t precision;        /* SR_PREC_* */
-  int32_t omega_per_lane;   /* 0 = tuned; split direct: 1, 2, 4 (far) / 1, 2 (near);
-                               recurrence: 2, 4 (far) / 2 (near); f64: 1 */
+  int32_t omega_per_lane;   /* 0 = tuned, else 1, 2 or 4; a value the variant lacks
+                               falls back: split direct 1, 2, 4 far / 1, 2 near, else 1;
+                               recurrence 2, 4 far / 2 near, else 2; f64 1, 2, else 1 */
   int32_t waves_per_block;  /* 0 = tuned, else 1, 2 or 4 */
   int32_t slices;           /* 0 = tuned, else 1..32 particle slices */
   int32_t persistent;       /* 0 = one block per tile; >0 = that many blocks loop over tiles */

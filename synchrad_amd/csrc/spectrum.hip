@@ -1339,9 +1339,9 @@ Plan make_plan(const sr_grid &g, const sr_opts &o, int64_t n_particles, uint32_t
     pl.sweep = SWEEP_REC;
   int best_w = 1, best_m = 1;
   double best_score = -1;
-  const int m_opts[4] = {1, 2, 4, 8};
+  const int m_opts[3] = {1, 2, 4};
   for (int w = 1; w <= 4; w *= 2) {
-    for (int mi = 0; mi < 4; mi++) {
+    for (int mi = 0; mi < 3; mi++) {
       const int m = m_opts[mi];
       if (!allowed_m(g.mode, o.precision, pl.sweep, m)) continue;
       const int64_t run = (int64_t)kWave * w * m;
@@ -1463,8 +1463,8 @@ int check_opts(const sr_grid *g, const sr_opts *o) {
   if (o->precision != SR_PREC_F64 && o->precision != SR_PREC_SPLIT)
     return fail(SR_ERR_INVALID, "precision %d is not F64 (0) or SPLIT (1)", o->precision);
   if (!(o->omega_per_lane == 0 || o->omega_per_lane == 1 || o->omega_per_lane == 2 ||
-        o->omega_per_lane == 4 || o->omega_per_lane == 8))
-    return fail(SR_ERR_INVALID, "omega_per_lane must be 0, 1, 2, 4 or 8");
+        o->omega_per_lane == 4))
+    return fail(SR_ERR_INVALID, "omega_per_lane must be 0, 1, 2 or 4");
   if (!(o->waves_per_block == 0 || o->waves_per_block == 1 || o->waves_per_block == 2 ||
         o->waves_per_block == 4))
     return fail(SR_ERR_INVALID, "waves_per_block must be 0, 1, 2 or 4");

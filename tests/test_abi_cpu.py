@@ -99,6 +99,7 @@ def _err():
     (dict(opts=dict(precision=5)), "precision"),
     (dict(opts=dict(waves_per_block=3)), "waves_per_block"),
     (dict(opts=dict(omega_per_lane=3)), "omega_per_lane"),
+    (dict(opts=dict(omega_per_lane=8)), "omega_per_lane"),
     (dict(opts=dict(sweep=5)), "sweep"),
     (dict(opts=dict(sweep=2)), "recurrence"),
     (dict(grid=dict(omega_step=float("inf"))), "omega_step"),
